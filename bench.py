@@ -1,0 +1,183 @@
+"""Throughput of the PhyloCSF scoring path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 58mammals tree
+and ECMs, 10,000 synthetic regions with lengths U{50..2000} codons (half
+simulated under the coding model, half under the noncoding model), each
+species-codon gapped with probability 0.1 and 1% of codons holding an N,
+fixed strategy.  One step = one batched forward over all ~10.25 M codon
+columns under both models (FP32 MFMA pruning with FP64 P(t) and FP64
+log-scaling, per-region FP64 reduction, results copied to the host).
+Inputs are resident in HBM before the timed region.
+
+Printed: one JSON line with value = codon-columns/s of the batched MFMA path,
+the plain FP64 GPU kernel's rate on the same batch, the accuracy of the
+FP32 scores against the FP64 kernel, the roofline of the dominant kernel
+(flops from HIP events on the kernel's own stream) and the FP64 CPU oracle's
+rate on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "codon-columns/s, batched MFMA vs plain FP64 GPU, at |delta score|<=1e-3 dB"
+PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(wl, seconds):
+    """FP64 oracle (C restatement of the reference loop), 1 thread, on the first
+    regions of the same workload until `seconds` of CPU work are done."""
+    from oracle import oracle as O
+    from phylocsf_amd.api import paramset_prefix
+    prefix = paramset_prefix("58mammals")
+    t = O.newick_tree(open(prefix + ".nh").read())
+    Sc, pc = O.ecm_read(prefix + "_coding.ECM")
+    Sn, pn = O.ecm_read(prefix + "_noncoding.ECM")
+    om = O.Model(t, [Sc, Sn], [pc, pn])
+    cols = 0
+    regions = 0
+    t0 = time.perf_counter()
+    for r in range(wl.n_regions):
+        lo, hi = int(wl.offsets[r]), int(wl.offsets[r + 1])
+        om.score_fixed(wl.codes[lo:hi])
+        cols += hi - lo
+        regions += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=cols / dt, unit="codon-columns/s", cores=1, kind="port",
+                sample="first %d regions (%d codon columns) of the workload, fixed strategy, "
+                       "FP64 oracle, %.1f s on 1 thread of %s" % (regions, cols, dt, cpu_model()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--regions", type=int, default=None, help="override region count")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (regions shard "
+              "across GPUs by design only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    import phylocsf_amd as P
+    from phylocsf_amd import workloads as W
+
+    ctx = P.Context(0)
+    ctx.set_timing(True)
+    model = W.model_for("C2", ctx)
+    log("simulating C2 ...")
+    t0 = time.perf_counter()
+    wl = W.make("C2", model, n_regions=args.regions, n_threads=16)
+    log("simulated %d regions, %d columns in %.1f s" % (wl.n_regions, wl.n_cols,
+                                                         time.perf_counter() - t0))
+    batch = P.Batch(model, wl.codes, wl.offsets)
+    nl = model.tree.nl
+    ncols = wl.n_cols
+
+    # ---- batched FP32 MFMA path (the measured value)
+    for _ in range(args.warmup):
+        batch.forward(1.0, P.PCSF_FP32)
+    ctx.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ll32, el32 = batch.forward(1.0, P.PCSF_FP32)
+        kernel_ms.append(batch.timing()["prune_ms"])
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = ncols * args.steps / elapsed
+    ms_per_step = 1000 * elapsed / args.steps
+    log("FP32 MFMA: %.3f ms/step, %.3e col/s, kernel %.3f ms" % (ms_per_step, value,
+                                                                np.mean(kernel_ms)))
+
+    # ---- plain FP64 GPU kernel on the same batch (the 50x denominator)
+    batch.forward(1.0, P.PCSF_FP64)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        ll64, el64 = batch.forward(1.0, P.PCSF_FP64)
+    ctx.synchronize()
+    fp64_rate = ncols * args.fp64_steps / (time.perf_counter() - t0)
+    log("FP64 plain: %.3e col/s" % fp64_rate)
+
+    s32 = P.DB * (ll32[:, 0] - ll32[:, 1])
+    s64 = P.DB * (ll64[:, 0] - ll64[:, 1])
+    dmax = float(np.max(np.abs(s32 - s64))) if len(s32) else 0.0
+
+    # ---- roofline of the dominant kernel (fused pruning: MFMA-bound; its HBM
+    # traffic is the codes plus per-column results, ~nl + 32 B per column)
+    flops_per_col = 16384.0 * (nl - 2)          # 2 models x (L-2) x 2*64^2
+    kavg_s = float(np.mean(kernel_ms)) / 1000.0
+    achieved = flops_per_col * ncols / kavg_s / 1e12
+    roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / PEAK_FP32_TFLOPS, traffic=None,
+                    kernel="prune_mfma_kernel", kernel_ms=float(np.mean(kernel_ms)),
+                    flops_per_column=flops_per_col)
+
+    cpu = None
+    if rank == 0 and args.cpu_seconds > 0:
+        cpu = cpu_baseline(wl, args.cpu_seconds)
+        log("CPU oracle: %.1f col/s" % cpu["value"])
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "codon-columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "58mammals, %d regions, lengths U{50..2000}, 50/50 coding/noncoding, "
+                        "gap p=0.1, 1%% N, fixed strategy (seed 2)" % wl.n_regions,
+            "codon_columns": ncols,
+            "models": 2,
+            "precision": "FP32 MFMA, FP64 P(t) and log-scaling",
+        },
+        "plain_fp64_gpu": {"value": fp64_rate, "unit": "codon-columns/s",
+                           "speedup_of_value": value / fp64_rate},
+        "accuracy": {"max_abs_delta_score_db_vs_fp64": dmax, "bound_db": 1e-3},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if rank == 0:
+        print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

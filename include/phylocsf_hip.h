@@ -1,0 +1,162 @@
+/*
+ * phylocsf_hip.h -- C ABI of the MI355X-native PhyloCSF scoring path.
+ *
+ * The boundary sits where the reference's per-column loop sits: one batched
+ * call replaces the column loop of PhyloCSFModel.lpr_leaves
+ * (src/PhyloCSFModel.ml:67-82) and the PhyloLik calls it makes
+ * (lib/CamlPaml/PhyloLik.ml:46-138); the P(t) builder replaces the to_Pt
+ * loop of PhyloModel.make (lib/CamlPaml/PhyloModel.ml:12-25,
+ * lib/CamlPaml/Q.ml:211-256).  Paths are relative to the mlin/PhyloCSF tree.
+ *
+ * Conventions
+ *   - every call returns int: 0 = ok, or one of the PCSF_E* codes below,
+ *     which mirror the reference's exceptions (Invalid_argument -> EINVAL,
+ *     Failure -> EFAIL); pcsf_last_error(ctx) gives the message.
+ *   - no C++ exceptions cross this boundary; no torch types appear in it.
+ *   - one context per host thread; a context owns one HIP stream (default
+ *     priority) and the device memory of its models and batches.
+ *   - trees use the reference's T numbering (lib/CamlPaml/T.mli:1-40):
+ *     leaves 0..nl-1 in left-to-right order, internal nodes in post-order,
+ *     root n-1; children[(i-nl)*2 + {0,1}] = (left, right) of internal i.
+ *   - leaf codes: codon index 16*n1+4*n2+n3 (lib/CamlPaml/Code.ml:140-141)
+ *     or PCSF_MARGINALIZE (255) for `Marginalize.
+ */
+#ifndef PHYLOCSF_HIP_H
+#define PHYLOCSF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCSF_OK 0
+#define PCSF_EINVAL (-1)  /* Invalid_argument in the reference */
+#define PCSF_EFAIL (-2)   /* Failure in the reference (numerics, parse) */
+#define PCSF_EDEVICE (-3) /* HIP runtime error */
+#define PCSF_ENOMEM (-4)  /* device or host allocation failed */
+
+#define PCSF_MARGINALIZE 255
+
+/* arithmetic of the pruning contraction */
+#define PCSF_FP64 0 /* plain FP64 kernel: one column per workgroup (GEMV) */
+#define PCSF_FP32 1 /* fused whole-tree FP32 MFMA kernel, FP64 log-scaling */
+
+typedef struct pcsf_ctx pcsf_ctx;
+typedef struct pcsf_model pcsf_model;
+typedef struct pcsf_batch pcsf_batch;
+
+/* ---- context --------------------------------------------------------- */
+
+int pcsf_ctx_create(int device, pcsf_ctx **out);
+void pcsf_ctx_destroy(pcsf_ctx *ctx);
+const char *pcsf_last_error(const pcsf_ctx *ctx);
+/* 1: record HIP events around every kernel of pcsf_forward (for timing) */
+int pcsf_ctx_set_timing(pcsf_ctx *ctx, int enable);
+/* the HIP stream (hipStream_t) every call of this context is ordered on */
+void *pcsf_ctx_stream(pcsf_ctx *ctx);
+int pcsf_ctx_synchronize(pcsf_ctx *ctx);
+
+/* ---- host-side parsing helpers (no device work) ----------------------- */
+
+/* ECM.import_parameters (src/ECM.ml:18-72): S[64*64] row-major symmetric,
+ * pi[64].  err may be NULL. */
+int pcsf_ecm_read(const char *path, double *S, double *pi, char *err, int errlen);
+
+/* NewickParser.parse (lib/CamlPaml/NewickParser.mly:7-24) + optional
+ * Newick.subtree (lib/CamlPaml/Newick.ml:34-41, keep_csv = "sp1,sp2,...")
+ * + T.of_newick (lib/CamlPaml/T.ml:57-112).  Arrays sized max_nodes;
+ * children is (n-nl)*2; branch[n] (NaN where unspecified); labels are
+ * written '\n'-separated into labels_buf.  *n_out = node count. */
+int pcsf_newick_tree(const char *text, const char *keep_csv, int max_nodes, int *n_out,
+                     int32_t *children, double *branch, char *labels_buf, int labels_len,
+                     char *err, int errlen);
+
+/* ---- model: tree + rate models --------------------------------------- */
+
+/* PhyloCSFModel.make generalised to n_models ECMs on one tree
+ * (src/PhyloCSFModel.ml:11-56,107-110; lib/CamlPaml/PhyloModel.ml:94-106):
+ * Q_m = (S_m o pi_m)/scale_m, root prior pi_m, branch lengths rho * b.
+ * Eigendecomposes each Q_m (FP64, host) and uploads it.
+ * S is n_models*64*64, pi is n_models*64, branch has n_nodes entries (the
+ * root's is ignored). */
+int pcsf_model_create(pcsf_ctx *ctx, int n_nodes, const int32_t *children,
+                      const double *branch, int n_models, const double *S, const double *pi,
+                      pcsf_model **out);
+void pcsf_model_destroy(pcsf_model *m);
+int pcsf_model_info(const pcsf_model *m, int *n_nodes, int *n_leaves, int *n_models,
+                    int *slots_needed);
+
+/* P(t) for every branch of model `model` at tree scale rho, as the reference
+ * builds it (Q.ml:211-249: eigen form, clamp |neg|<=1e-6 to 0, diagonal
+ * 1 - sum(offdiag)); built on the GPU in FP64.  P_out is (n_nodes-1)*64*64
+ * row-major FP64 (host). */
+int pcsf_model_transition(pcsf_model *m, int model, double rho, double *P_out);
+
+/* ---- batch: packed columns of many regions ---------------------------- */
+
+/* codes: n_cols x n_leaves (row per codon column, as PhyloCSF.pleaves
+ * yields them, src/PhyloCSF.ml:219-246), host memory; uploaded once and
+ * kept leaf-major in HBM.  region_offsets[n_regions+1]: columns of region r
+ * are [off[r], off[r+1]). */
+int pcsf_batch_create(pcsf_model *m, const uint8_t *codes, int64_t n_cols,
+                      const int64_t *region_offsets, int32_t n_regions, pcsf_batch **out);
+void pcsf_batch_destroy(pcsf_batch *b);
+int64_t pcsf_batch_columns(const pcsf_batch *b);
+
+/* Forward: for every region r and model m,
+ *   loglik[r*n_models+m] = sum_col ln z_col          (lpr_leaves)
+ *   elpr[r*n_models+m]   = sum_col post_root . ln pi (elpr_anc)
+ * rho: n_models values (every region shares them; the fixed strategy passes
+ * 1.0) or, with rho_per_region = 1, n_regions*n_models values.
+ * loglik / elpr may be NULL (results stay on the device).
+ * save_for_backward: keep what pcsf_backward needs (FP64 path only). */
+int pcsf_forward(pcsf_batch *b, const double *rho, int rho_per_region, int precision,
+                 double *loglik, double *elpr, int save_for_backward);
+
+/* Backward of the last forward: given dL/dloglik (n_regions*n_models),
+ * returns dL/drho (n_regions*n_models) and optionally dL/dt_br summed over
+ * the batch (n_models*(n_nodes-1)).  The gradient the reference's PhyloEM
+ * defines (lib/CamlPaml/PhyloEM.ml:140-182, d ln z/dt through branch
+ * posteriors, PhyloLik.ml:140-174). */
+int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
+                  double *grad_branch);
+
+/* Device pointers of the last forward's per-region results (FP64,
+ * n_regions*n_models each) and the kernel times when timing is enabled. */
+int pcsf_batch_device_results(pcsf_batch *b, const double **d_loglik, const double **d_elpr);
+int pcsf_batch_last_timing(const pcsf_batch *b, double *prune_ms, double *reduce_ms,
+                           double *pbuild_ms);
+
+/* ---- PhyloLik compatibility entry ------------------------------------ */
+
+/* PhyloLik.prepare + likelihood + node_posterior (lib/CamlPaml/PhyloLik.ml:
+ * 46-138) over n_cols columns with caller-supplied P matrices, any alphabet
+ * size k <= 64:
+ *   P[(n_nodes-1)*k*k] row-major, P[br] for the branch leading to node br;
+ *   prior[k]; leaves[n_cols*nl] state index or -1 (`Marginalize).
+ * Outputs z[n_cols] and, if non-NULL, root_posterior[n_cols*k].  FP64. */
+int pcsf_lik_columns(pcsf_ctx *ctx, int k, int n_nodes, const int32_t *children,
+                     const double *P, const double *prior, const int32_t *leaves,
+                     int64_t n_cols, double *z, double *root_posterior);
+
+/* ---- synthetic alignments (bench / tests) ----------------------------- */
+
+/* Simulates codes for packed regions: region r is drawn from model
+ * region_model[r] at tree scale region_rho[r]: root ~ pi, each child ~ row
+ * of P(rho*b) (lib/CamlPaml/PhyloModel.ml:38-54).  Leaves 1..nl-1 are then
+ * masked to PCSF_MARGINALIZE: gap_mode 0 = i.i.d. with probability gap_p,
+ * gap_mode 1 = runs of mean length gap_run (codons) covering gap_p; and
+ * with probability n_p a codon holds an N (also `Marginalize).
+ * Counter-based Philox4x32-10 streams keyed by (seed, region, column): the
+ * result does not depend on the thread count.  codes_out is n_cols*nl. */
+int pcsf_simulate(pcsf_model *m, int32_t n_regions, const int64_t *region_offsets,
+                  const int32_t *region_model, const double *region_rho, int gap_mode,
+                  double gap_p, double gap_run, double n_p, uint64_t seed, int n_threads,
+                  uint8_t *codes_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

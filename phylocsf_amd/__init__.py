@@ -1,0 +1,14 @@
+"""MI355X-native PhyloCSF scoring path (HIP kernels for gfx950 behind a C ABI).
+
+The public surface mirrors the reference's CamlPaml / PhyloCSF interface; see
+api.py.  The HIP library (lib/libphylocsf_amd.so) is required: nothing here
+falls back to a CPU implementation.
+"""
+from ._lib import (PCSF_FP32, PCSF_FP64, PCSF_MARGINALIZE, Failure, InvalidArgument,  # noqa
+                   PcsfError)
+from .api import (DB, Batch, Context, PhyloCSFModel, PhyloLik, Tree, import_parameters,  # noqa
+                  newick_tree, paramset_prefix, score_fixed)
+
+__all__ = ["Context", "PhyloCSFModel", "Batch", "PhyloLik", "Tree", "import_parameters",
+           "newick_tree", "paramset_prefix", "score_fixed", "DB", "PCSF_FP32", "PCSF_FP64",
+           "PCSF_MARGINALIZE", "PcsfError", "InvalidArgument", "Failure"]

@@ -1,0 +1,310 @@
+"""Host-side mirror of the reference's scoring interface over the HIP C ABI.
+
+Names follow the reference so a caller of CamlPaml / PhyloCSF finds the
+same entry points (paths relative to the mlin/PhyloCSF tree):
+
+  import_parameters ........ src/ECM.ml:18-72
+  newick_tree .............. NewickParser + Newick.subtree + T.of_newick
+  PhyloCSFModel ............ src/PhyloCSFModel.ml:107-146 (make / lpr_leaves / score)
+  PhyloLik ................. lib/CamlPaml/PhyloLik.mli:6-35 (prepare / likelihood /
+                             node_posterior at the root), batched over columns
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from ._lib import (PCSF_FP32, PCSF_FP64, PCSF_MARGINALIZE, InvalidArgument, lib, raise_for)
+
+DB = 10.0 / np.log(10.0)  # PhyloCSFModel.db (src/PhyloCSFModel.ml:113)
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PARAMS_DIR = os.path.join(_HERE, "data", "PhyloCSF_Parameters")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def _u8(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def import_parameters(path: str):
+    """ECM.import_parameters: -> (S [64,64], pi [64])."""
+    S = np.zeros((64, 64))
+    pi = np.zeros(64)
+    err = C.create_string_buffer(512)
+    rc = lib().pcsf_ecm_read(path.encode(), _dp(S), _dp(pi), err, 512)
+    if rc:
+        raise_for(rc, err.value.decode())
+    return S, pi
+
+
+@dataclass
+class Tree:
+    """T.t: leaves 0..nl-1 left to right, post-order internals, root n-1."""
+    n: int
+    children: np.ndarray  # (n-nl, 2) int32
+    branch: np.ndarray    # (n,) float64, NaN where unspecified
+    labels: List[str]
+
+    @property
+    def nl(self):
+        return (self.n + 1) // 2
+
+    def parent(self):
+        p = np.full(self.n, -1, dtype=np.int32)
+        for i in range(self.nl, self.n):
+            l, r = self.children[i - self.nl]
+            p[l] = p[r] = i
+        return p
+
+
+def newick_tree(text: str, species: Optional[Sequence[str]] = None, max_nodes=1 << 15) -> Tree:
+    """NewickParser.parse + optional Newick.subtree + T.of_newick."""
+    n = C.c_int(0)
+    ch = np.zeros(max_nodes, dtype=np.int32)
+    br = np.zeros(max_nodes)
+    lab = C.create_string_buffer(1 << 20)
+    err = C.create_string_buffer(512)
+    keep = None if species is None else ",".join(species).encode()
+    rc = lib().pcsf_newick_tree(text.encode(), keep, max_nodes, C.byref(n), _i32(ch), _dp(br),
+                                lab, 1 << 20, err, 512)
+    if rc:
+        raise_for(rc, err.value.decode())
+    m = n.value
+    nl = (m + 1) // 2
+    return Tree(m, ch[:2 * (m - nl)].reshape(m - nl, 2).copy(), br[:m].copy(),
+                lab.value.decode().split("\n"))
+
+
+def paramset_prefix(paramset: str) -> str:
+    """Parameter-set lookup (src/PhyloCSF.ml:407-421): a bare name resolves under
+    $PHYLOCSF_BASE/PhyloCSF_Parameters (default: the copy shipped in this
+    package); a name containing '/' is a path prefix."""
+    if "/" in paramset:
+        return paramset
+    base = os.environ.get("PHYLOCSF_BASE")
+    if base:
+        if not os.path.isdir(base):
+            raise RuntimeError("PHYLOCSF_BASE environment variable must be set to the root "
+                               "directory of the source or executable distribution")
+        return os.path.join(base, "PhyloCSF_Parameters", paramset)
+    return os.path.join(PARAMS_DIR, paramset)
+
+
+class Context:
+    """One HIP stream + the device memory of its models and batches."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().pcsf_ctx_create(device, C.byref(h))
+        if rc:
+            raise_for(rc, "pcsf_ctx_create(device=%d) failed" % device)
+        self.h = h
+
+    def check(self, rc):
+        if rc:
+            raise_for(rc, lib().pcsf_last_error(self.h).decode(errors="replace"))
+
+    def set_timing(self, on=True):
+        self.check(lib().pcsf_ctx_set_timing(self.h, 1 if on else 0))
+
+    def synchronize(self):
+        self.check(lib().pcsf_ctx_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pcsf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PhyloCSFModel:
+    """PhyloCSFModel.make over one tree and n ECMs (0 = coding, 1 = noncoding)."""
+
+    def __init__(self, ctx: Context, tree: Tree, ecms: Sequence):
+        self.ctx = ctx
+        self.tree = tree
+        S = np.ascontiguousarray(np.stack([e[0] for e in ecms]), dtype=np.float64)
+        pi = np.ascontiguousarray(np.stack([e[1] for e in ecms]), dtype=np.float64)
+        self.pi = pi
+        self.nm = len(ecms)
+        ch = np.ascontiguousarray(tree.children, dtype=np.int32)
+        br = np.ascontiguousarray(tree.branch, dtype=np.float64).copy()
+        br[-1] = 0.0  # the root branch is unused
+        h = C.c_void_p()
+        ctx.check(lib().pcsf_model_create(ctx.h, tree.n, _i32(ch), _dp(br), self.nm, _dp(S),
+                                          _dp(pi), C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_paramset(cls, ctx: Context, paramset: str, species=None):
+        prefix = paramset_prefix(paramset)
+        tree = newick_tree(open(prefix + ".nh").read(), species)
+        return cls(ctx, tree, [import_parameters(prefix + "_coding.ECM"),
+                               import_parameters(prefix + "_noncoding.ECM")])
+
+    def info(self):
+        n, nl, nm, sl = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self.ctx.check(lib().pcsf_model_info(self.h, C.byref(n), C.byref(nl), C.byref(nm),
+                                             C.byref(sl)))
+        return dict(n=n.value, nl=nl.value, nm=nm.value, slots=sl.value)
+
+    def transition(self, model: int, rho: float = 1.0):
+        """P(rho*b) for every branch, built on the GPU (Q.Diag.to_Pt)."""
+        P = np.zeros((self.tree.n - 1, 64, 64))
+        self.ctx.check(lib().pcsf_model_transition(self.h, model, float(rho), _dp(P)))
+        return P
+
+    def batch(self, regions: Sequence[np.ndarray]) -> "Batch":
+        return Batch.from_regions(self, regions)
+
+    def simulate(self, lengths, region_model, region_rho, gap_mode=0, gap_p=0.0, gap_run=5.0,
+                 n_p=0.0, seed=1, n_threads=0):
+        """Synthetic packed codes [n_cols, nl] and region offsets."""
+        lengths = np.asarray(lengths, dtype=np.int64)
+        off = np.zeros(len(lengths) + 1, dtype=np.int64)
+        np.cumsum(lengths, out=off[1:])
+        codes = np.zeros((int(off[-1]), self.tree.nl), dtype=np.uint8)
+        rm = np.ascontiguousarray(region_model, dtype=np.int32)
+        rr = np.ascontiguousarray(region_rho, dtype=np.float64)
+        rc = lib().pcsf_simulate(self.h, len(lengths), _i64(off), _i32(rm), _dp(rr), gap_mode,
+                                 gap_p, gap_run, n_p, seed, n_threads, _u8(codes))
+        if rc:
+            raise_for(rc, "pcsf_simulate failed")
+        return codes, off
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pcsf_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Packed codon columns of many regions, resident in HBM."""
+
+    def __init__(self, model: PhyloCSFModel, codes: np.ndarray, offsets: np.ndarray):
+        self.model = model
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        if codes.ndim != 2 or codes.shape[1] != model.tree.nl:
+            raise InvalidArgument(-1, "codes must be [n_cols, n_leaves]")
+        self.n_regions = len(offsets) - 1
+        self.n_cols = codes.shape[0]
+        h = C.c_void_p()
+        model.ctx.check(lib().pcsf_batch_create(model.h, _u8(codes), self.n_cols, _i64(offsets),
+                                                self.n_regions, C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_regions(cls, model, regions):
+        lens = [r.shape[0] for r in regions]
+        off = np.zeros(len(regions) + 1, dtype=np.int64)
+        np.cumsum(lens, out=off[1:])
+        codes = (np.concatenate(regions, axis=0) if regions
+                 else np.zeros((0, model.tree.nl), np.uint8))
+        return cls(model, codes, off)
+
+    def forward(self, rho=1.0, precision=PCSF_FP32, per_region=False):
+        """-> (loglik [R, nm], elpr [R, nm]): lpr_leaves for every region/model."""
+        nm = self.model.nm
+        if per_region:
+            r = np.ascontiguousarray(rho, dtype=np.float64).reshape(self.n_regions * nm)
+        else:
+            r = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, dtype=np.float64), (nm,)))
+        ll = np.zeros((self.n_regions, nm))
+        el = np.zeros((self.n_regions, nm))
+        self.model.ctx.check(lib().pcsf_forward(self.h, _dp(r), 1 if per_region else 0,
+                                                precision, _dp(ll), _dp(el), 0))
+        return ll, el
+
+    def timing(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self.model.ctx.check(lib().pcsf_batch_last_timing(self.h, C.byref(a), C.byref(b),
+                                                          C.byref(c)))
+        return dict(prune_ms=a.value, reduce_ms=b.value, pbuild_ms=c.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pcsf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def score_fixed(batch: Batch, precision=PCSF_FP32):
+    """llr_FixedLik 1.0 per region (src/PhyloCSFModel.ml:122-128):
+    -> dict of arrays score, anc_comp, lpr_c, lpr_n (decibans for the first two)."""
+    ll, el = batch.forward(1.0, precision)
+    return dict(score=DB * (ll[:, 0] - ll[:, 1]), anc_comp=DB * (el[:, 0] - el[:, 1]),
+                lpr_c=ll[:, 0], lpr_n=ll[:, 1])
+
+
+class PhyloLik:
+    """CamlPaml.PhyloLik over many columns at once (lib/CamlPaml/PhyloLik.mli):
+    prepare(tree, pms, prior, leaves) then likelihood / node_posterior(root)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def prepare(self, tree: Tree, pms: np.ndarray, prior: np.ndarray, leaves: np.ndarray):
+        k = len(prior)
+        leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+        if leaves.ndim == 1:
+            leaves = leaves[None, :]
+        if leaves.shape[1] != tree.nl:
+            raise InvalidArgument(-1, "CamlPaml.Infer.prepare: length(leaves) != leaves(t)")
+        if pms.shape[0] < tree.n - 1:
+            raise InvalidArgument(-1, "CamlPaml.Infer.prepare: not enough P matrices")
+        P = np.ascontiguousarray(pms[:tree.n - 1], dtype=np.float64)
+        pr = np.ascontiguousarray(prior, dtype=np.float64)
+        ncols = leaves.shape[0]
+        z = np.zeros(ncols)
+        post = np.zeros((ncols, k))
+        ch = np.ascontiguousarray(tree.children, dtype=np.int32)
+        self.ctx.check(lib().pcsf_lik_columns(self.ctx.h, k, tree.n, _i32(ch), _dp(P), _dp(pr),
+                                              _i32(leaves), ncols, _dp(z), _dp(post)))
+        return _Intermediate(tree, z, post)
+
+
+@dataclass
+class _Intermediate:
+    tree: Tree
+    z: np.ndarray
+    root_post: np.ndarray
+
+    def likelihood(self):
+        return self.z
+
+    def node_posterior(self, i):
+        if i != self.tree.n - 1:
+            raise InvalidArgument(-1, "node_posterior: only the root is served by this entry")
+        return self.root_post

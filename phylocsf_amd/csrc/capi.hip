@@ -1,0 +1,600 @@
+// capi.hip -- the extern "C" boundary (include/phylocsf_hip.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <new>
+
+#include "host/text.hpp"
+#include "internal.hpp"
+
+namespace {
+
+int set_err(pcsf_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return set_err((ctx), e_ == hipErrorOutOfMemory ? PCSF_ENOMEM : PCSF_EDEVICE,     \
+                     std::string(#expr) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return hipSuccess;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// Compile the tree into the fused kernel's op program: post-order, the
+// subtree needing more live tiles first (Sethi-Ullman order), so the number
+// of live 64x32 tiles is minimal.
+struct ProgramBuilder {
+  const std::vector<int>& lc;
+  const std::vector<int>& rc;
+  int nl;
+  std::vector<int> need;
+  std::vector<int32_t> ops;
+  int max_depth = 0;
+  std::vector<int> gemm_pos;  // index into ops (pairs) of each GEMM
+
+  ProgramBuilder(const std::vector<int>& l, const std::vector<int>& r, int nleaves)
+      : lc(l), rc(r), nl(nleaves), need(l.size(), 0) {}
+
+  bool leaf(int i) const { return i < nl; }
+
+  void compute_need() {
+    for (int i = nl; i < static_cast<int>(lc.size()); ++i) {
+      const int a = lc[i], b = rc[i];
+      if (leaf(a) && leaf(b)) need[i] = 1;
+      else if (leaf(a)) need[i] = std::max(1, need[b]);
+      else if (leaf(b)) need[i] = std::max(1, need[a]);
+      else {
+        const int x = need[a], y = need[b];
+        need[i] = std::max(std::max(x, y), std::min(x, y) + 1);
+      }
+    }
+  }
+
+  void emit(int op, int d, int a0, int a1) {
+    ops.push_back(op | (d << 8));
+    ops.push_back((a0 & 0xffff) | ((a1 & 0xffff) << 16));
+    max_depth = std::max(max_depth, d);
+  }
+
+  void gemm(int d, int br) {
+    gemm_pos.push_back(static_cast<int>(ops.size() / 2));
+    emit(OP_GEMM, d, br, kNoBranch);
+  }
+
+  void walk(int i, int d) {
+    const int a = lc[i], b = rc[i];
+    if (leaf(a) && leaf(b)) {
+      emit(OP_CHERRY, d, a, b);
+    } else if (leaf(a) || leaf(b)) {
+      const int x = leaf(a) ? a : b, y = leaf(a) ? b : a;
+      walk(y, d);
+      gemm(d, y);
+      emit(OP_LEAFMUL, d, x, 0);
+    } else {
+      const int first = (need[b] > need[a]) ? b : a, second = (first == a) ? b : a;
+      walk(first, d);
+      gemm(d, first);
+      walk(second, d + 1);
+      gemm(d + 1, second);
+      emit(OP_MULPOP, d, 0, 0);
+    }
+  }
+
+  void build(int root) {
+    compute_need();
+    walk(root, 0);
+    emit(OP_ROOT, 0, 0, 0);
+    emit(OP_END, 0, 0, 0);
+    // each GEMM prefetches the next GEMM's tile
+    for (size_t k = 0; k + 1 < gemm_pos.size(); ++k) {
+      int32_t& w1 = ops[2 * gemm_pos[k] + 1];
+      const int br = w1 & 0xffff;
+      const int nbr = ops[2 * gemm_pos[k + 1] + 1] & 0xffff;
+      w1 = br | (nbr << 16);
+    }
+  }
+};
+
+bool rho_valid(double r) { return r > 0.0 && std::isfinite(r); }
+
+int ensure_transitions(pcsf_model* m, const std::vector<double>& rho, float* ms_out) {
+  pcsf_ctx* c = m->ctx;
+  if (m->cached_rho == rho) {
+    if (ms_out) *ms_out = 0.f;
+    return PCSF_OK;
+  }
+  HIP_TRY(c, hipMemcpyAsync(m->d_rho, rho.data(), sizeof(double) * rho.size(),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemsetAsync(m->d_perr, 0, sizeof(int) * 8, c->stream));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing && ms_out) {
+    HIP_TRY(c, hipEventCreate(&e0));
+    HIP_TRY(c, hipEventCreate(&e1));
+    HIP_TRY(c, hipEventRecord(e0, c->stream));
+  }
+  HIP_TRY(c, launch_pbuild(m, m->d_rho, c->stream));
+  if (e1) HIP_TRY(c, hipEventRecord(e1, c->stream));
+  int perr[8];
+  double perrval = 0;
+  HIP_TRY(c, hipMemcpyAsync(perr, m->d_perr, sizeof perr, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(&perrval, m->d_perrval, sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (e0) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_out = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  if (perr[0] != 0) {
+    m->cached_rho.clear();
+    char buf[256];
+    const int br = perr[1], q = perr[2];
+    const double t = rho[q] * m->branch[br];
+    switch (perr[0]) {
+      case 1:
+        std::snprintf(buf, sizeof buf,
+                      "CamlPaml.Q.substition_matrix: expm(%.2e*Q)[%d,%d] = %e < 0", t, perr[3],
+                      perr[4], perrval);
+        break;
+      case 2:
+        std::snprintf(buf, sizeof buf,
+                      "CamlPaml.Q.substitution matrix: sum(expm(%.2e*Q)[%d,] = %e > 1", t,
+                      perr[3], perrval);
+        break;
+      case 4:
+        return set_err(c, PCSF_EINVAL, "CamlPaml.PhyloModel.make");
+      default:
+        std::snprintf(buf, sizeof buf, "Assert_failure (Q.ml:245) row %d", perr[3]);
+    }
+    return set_err(c, PCSF_EFAIL, buf);
+  }
+  m->cached_rho = rho;
+  return PCSF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcsf_ctx_create(int device, pcsf_ctx** out) {
+  if (!out) return PCSF_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PCSF_EDEVICE;
+  if (device < 0 || device >= ndev) return PCSF_EINVAL;
+  auto* c = new (std::nothrow) pcsf_ctx();
+  if (!c) return PCSF_ENOMEM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return PCSF_EDEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    c->num_cus = prop.multiProcessorCount;
+  *out = c;
+  return PCSF_OK;
+}
+
+void pcsf_ctx_destroy(pcsf_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* pcsf_last_error(const pcsf_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int pcsf_ctx_set_timing(pcsf_ctx* c, int enable) {
+  if (!c) return PCSF_EINVAL;
+  c->timing = enable != 0;
+  return PCSF_OK;
+}
+
+void* pcsf_ctx_stream(pcsf_ctx* c) { return c ? reinterpret_cast<void*>(c->stream) : nullptr; }
+
+int pcsf_ctx_synchronize(pcsf_ctx* c) {
+  if (!c) return PCSF_EINVAL;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PCSF_OK;
+}
+
+int pcsf_ecm_read(const char* path, double* S, double* pi, char* err, int errlen) {
+  try {
+    pcsf::read_ecm(path, S, pi);
+  } catch (const pcsf::ParseError& e) {
+    if (err && errlen > 0) std::snprintf(err, errlen, "%s", e.msg.c_str());
+    return e.code;
+  }
+  return PCSF_OK;
+}
+
+int pcsf_newick_tree(const char* text, const char* keep_csv, int max_nodes, int* n_out,
+                     int32_t* children, double* branch, char* labels_buf, int labels_len,
+                     char* err, int errlen) {
+  try {
+    auto t = pcsf::parse_newick(text);
+    if (keep_csv) {
+      std::vector<std::string> keep;
+      std::string s(keep_csv);
+      size_t i = 0;
+      while (i <= s.size()) {
+        size_t j = s.find(',', i);
+        if (j == std::string::npos) j = s.size();
+        keep.push_back(s.substr(i, j - i));
+        i = j + 1;
+      }
+      t = pcsf::newick_subtree(std::move(t), keep);
+      if (!t || pcsf::newick_leaves(*t) <= 1)
+        throw pcsf::ParseError{PCSF_EFAIL, "specify at least two available --species"};
+    }
+    pcsf::Tree T = pcsf::tree_of_newick(*t);
+    if (T.n > max_nodes) throw pcsf::ParseError{PCSF_EINVAL, "tree larger than max_nodes"};
+    *n_out = T.n;
+    for (int i = T.nl; i < T.n; ++i) {
+      children[(i - T.nl) * 2] = T.lc[i];
+      children[(i - T.nl) * 2 + 1] = T.rc[i];
+    }
+    for (int i = 0; i < T.n; ++i) branch[i] = T.branch[i];
+    if (labels_buf && labels_len > 0) {
+      std::string all;
+      for (int i = 0; i < T.n; ++i) {
+        if (i) all += '\n';
+        all += T.labels[i];
+      }
+      if (static_cast<int>(all.size()) >= labels_len)
+        throw pcsf::ParseError{PCSF_EINVAL, "labels buffer too small"};
+      std::memcpy(labels_buf, all.c_str(), all.size() + 1);
+    }
+  } catch (const pcsf::ParseError& e) {
+    if (err && errlen > 0) std::snprintf(err, errlen, "%s", e.msg.c_str());
+    return e.code;
+  }
+  return PCSF_OK;
+}
+
+int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double* branch,
+                      int nm, const double* S, const double* pi, pcsf_model** out) {
+  if (!c || !out) return PCSF_EINVAL;
+  *out = nullptr;
+  if (n < 3 || n % 2 == 0 || !children || !branch || nm < 1 || nm > 8 || !S || !pi)
+    return set_err(c, PCSF_EINVAL, "CamlPaml.PhyloModel.make: bad tree or model count");
+  const int nl = (n + 1) / 2;
+  if (n - 1 >= kNoBranch) return set_err(c, PCSF_EINVAL, "tree too large");
+  auto m = new (std::nothrow) pcsf_model();
+  if (!m) return PCSF_ENOMEM;
+  std::unique_ptr<pcsf_model, void (*)(pcsf_model*)> guard(m, pcsf_model_destroy);
+  m->ctx = c;
+  m->n = n;
+  m->nl = nl;
+  m->nm = nm;
+  m->nbr = n - 1;
+  m->lc.assign(n, -1);
+  m->rc.assign(n, -1);
+  m->parent.assign(n, -1);
+  for (int i = nl; i < n; ++i) {
+    const int a = children[(i - nl) * 2], b = children[(i - nl) * 2 + 1];
+    if (a < 0 || b < 0 || a >= i || b >= i || a == b || m->parent[a] >= 0 || m->parent[b] >= 0)
+      return set_err(c, PCSF_EINVAL, "CamlPaml.T: children must precede parents (T order)");
+    m->lc[i] = a;
+    m->rc[i] = b;
+    m->parent[a] = i;
+    m->parent[b] = i;
+  }
+  for (int i = 0; i < n - 1; ++i)
+    if (m->parent[i] < 0) return set_err(c, PCSF_EINVAL, "CamlPaml.T: orphan node");
+  m->branch.assign(branch, branch + n - 1);
+  for (double bl : m->branch)
+    if (!(bl >= 0.0) || !std::isfinite(bl))
+      return set_err(c, PCSF_EINVAL, "CamlPaml.PhyloModel.make: negative or missing branch");
+  m->eig.resize(nm);
+  m->pi.assign(pi, pi + 64 * nm);
+  std::vector<double> V(nm * 4096), Vi(nm * 4096), lam(nm * 64), lnpi(nm * 64);
+  for (int q = 0; q < nm; ++q) {
+    std::vector<double> Q(4096);
+    std::string msg;
+    if (!pcsf::build_rate_matrix(64, S + q * 4096, pi + q * 64, Q.data(), &msg))
+      return set_err(c, PCSF_EFAIL, msg);
+    if (!pcsf::eigen_reversible(64, Q.data(), pi + q * 64, &m->eig[q], &msg))
+      return set_err(c, PCSF_EINVAL, msg);
+    std::copy(m->eig[q].V.begin(), m->eig[q].V.end(), V.begin() + q * 4096);
+    std::copy(m->eig[q].Vinv.begin(), m->eig[q].Vinv.end(), Vi.begin() + q * 4096);
+    std::copy(m->eig[q].lam.begin(), m->eig[q].lam.end(), lam.begin() + q * 64);
+    for (int x = 0; x < 64; ++x) lnpi[q * 64 + x] = std::log(pi[q * 64 + x]);
+  }
+  ProgramBuilder pb(m->lc, m->rc, nl);
+  pb.build(n - 1);
+  m->prog = pb.ops;
+  m->slots = pb.max_depth + 1;
+  m->n_gemm = static_cast<int>(pb.gemm_pos.size());
+  m->first_gemm_br = pb.gemm_pos.empty() ? kNoBranch : (m->prog[2 * pb.gemm_pos[0] + 1] & 0xffff);
+  if (m->slots > kMaxSlots)
+    return set_err(c, PCSF_EINVAL, "tree needs more than 6 live partial tiles");
+
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, dalloc(&m->d_V, V.size()));
+  HIP_TRY(c, dalloc(&m->d_Vinv, Vi.size()));
+  HIP_TRY(c, dalloc(&m->d_lam, lam.size()));
+  HIP_TRY(c, dalloc(&m->d_pi, m->pi.size()));
+  HIP_TRY(c, dalloc(&m->d_lnpi, lnpi.size()));
+  HIP_TRY(c, dalloc(&m->d_branch, m->branch.size()));
+  HIP_TRY(c, dalloc(&m->d_lc, (size_t)n));
+  HIP_TRY(c, dalloc(&m->d_rc, (size_t)n));
+  HIP_TRY(c, dalloc(&m->d_prog, m->prog.size()));
+  HIP_TRY(c, dalloc(&m->d_rho, (size_t)nm));
+  HIP_TRY(c, dalloc(&m->d_perr, (size_t)8));
+  HIP_TRY(c, dalloc(&m->d_perrval, (size_t)1));
+  const size_t pcount = (size_t)nm * m->nbr * 4096;
+  HIP_TRY(c, dalloc(&m->d_P64T, pcount));
+  HIP_TRY(c, dalloc(&m->d_PT32, pcount));
+  HIP_TRY(c, dalloc(&m->d_Pfrag, pcount));
+  std::vector<int32_t> lc32(m->lc.begin(), m->lc.end()), rc32(m->rc.begin(), m->rc.end());
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  HIP_TRY(c, up(m->d_V, V.data(), V.size() * 8));
+  HIP_TRY(c, up(m->d_Vinv, Vi.data(), Vi.size() * 8));
+  HIP_TRY(c, up(m->d_lam, lam.data(), lam.size() * 8));
+  HIP_TRY(c, up(m->d_pi, m->pi.data(), m->pi.size() * 8));
+  HIP_TRY(c, up(m->d_lnpi, lnpi.data(), lnpi.size() * 8));
+  HIP_TRY(c, up(m->d_branch, m->branch.data(), m->branch.size() * 8));
+  HIP_TRY(c, up(m->d_lc, lc32.data(), n * 4));
+  HIP_TRY(c, up(m->d_rc, rc32.data(), n * 4));
+  HIP_TRY(c, up(m->d_prog, m->prog.data(), m->prog.size() * 4));
+  *out = guard.release();
+  return PCSF_OK;
+}
+
+void pcsf_model_destroy(pcsf_model* m) {
+  if (!m) return;
+  dfree(m->d_V);
+  dfree(m->d_Vinv);
+  dfree(m->d_lam);
+  dfree(m->d_pi);
+  dfree(m->d_lnpi);
+  dfree(m->d_branch);
+  dfree(m->d_lc);
+  dfree(m->d_rc);
+  dfree(m->d_prog);
+  dfree(m->d_rho);
+  dfree(m->d_perr);
+  dfree(m->d_perrval);
+  dfree(m->d_P64T);
+  dfree(m->d_PT32);
+  dfree(m->d_Pfrag);
+  delete m;
+}
+
+int pcsf_model_info(const pcsf_model* m, int* n, int* nl, int* nm, int* slots) {
+  if (!m) return PCSF_EINVAL;
+  if (n) *n = m->n;
+  if (nl) *nl = m->nl;
+  if (nm) *nm = m->nm;
+  if (slots) *slots = m->slots;
+  return PCSF_OK;
+}
+
+int pcsf_model_transition(pcsf_model* m, int model, double rho, double* P_out) {
+  if (!m || !P_out) return PCSF_EINVAL;
+  pcsf_ctx* c = m->ctx;
+  if (model < 0 || model >= m->nm) return set_err(c, PCSF_EINVAL, "model index out of range");
+  if (!rho_valid(rho))
+    return set_err(c, PCSF_EINVAL, "CamlPaml.P14n.instantiate_tree: domain violation on variable 0");
+  std::vector<double> r(m->nm, rho);
+  int rc = ensure_transitions(m, r, nullptr);
+  if (rc) return rc;
+  std::vector<double> T((size_t)m->nbr * 4096);
+  HIP_TRY(c, hipMemcpy(T.data(), m->d_P64T + (size_t)model * m->nbr * 4096, T.size() * 8,
+                       hipMemcpyDeviceToHost));
+  for (int br = 0; br < m->nbr; ++br)
+    for (int a = 0; a < 64; ++a)
+      for (int b = 0; b < 64; ++b)
+        P_out[(size_t)br * 4096 + a * 64 + b] = T[(size_t)br * 4096 + b * 64 + a];
+  return PCSF_OK;
+}
+
+int pcsf_batch_create(pcsf_model* m, const uint8_t* codes, int64_t ncols,
+                      const int64_t* off, int32_t nreg, pcsf_batch** out) {
+  if (!m || !out) return PCSF_EINVAL;
+  pcsf_ctx* c = m->ctx;
+  *out = nullptr;
+  if (ncols < 0 || nreg < 0 || (ncols > 0 && !codes) || !off)
+    return set_err(c, PCSF_EINVAL, "pcsf_batch_create: bad arguments");
+  if (off[0] != 0 || off[nreg] != ncols)
+    return set_err(c, PCSF_EINVAL, "pcsf_batch_create: region offsets must span [0, n_cols]");
+  for (int32_t r = 0; r < nreg; ++r)
+    if (off[r + 1] < off[r])
+      return set_err(c, PCSF_EINVAL, "pcsf_batch_create: region offsets must not decrease");
+  auto b = new (std::nothrow) pcsf_batch();
+  if (!b) return PCSF_ENOMEM;
+  std::unique_ptr<pcsf_batch, void (*)(pcsf_batch*)> guard(b, pcsf_batch_destroy);
+  b->m = m;
+  b->ncols = ncols;
+  b->nreg = nreg;
+  b->ld = std::max<int64_t>(kGroupCols, (ncols + kGroupCols - 1) / kGroupCols * kGroupCols);
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, dalloc(&b->d_codes, (size_t)m->nl * b->ld));
+  HIP_TRY(c, dalloc(&b->d_offsets, (size_t)nreg + 1));
+  HIP_TRY(c, dalloc(&b->d_logz, (size_t)m->nm * b->ld));
+  HIP_TRY(c, dalloc(&b->d_colelpr, (size_t)m->nm * b->ld));
+  HIP_TRY(c, dalloc(&b->d_loglik, std::max<size_t>(1, (size_t)nreg * m->nm)));
+  HIP_TRY(c, dalloc(&b->d_elpr, std::max<size_t>(1, (size_t)nreg * m->nm)));
+  HIP_TRY(c, hipMemcpy(b->d_offsets, off, sizeof(int64_t) * (nreg + 1), hipMemcpyHostToDevice));
+  // upload in chunks through a staging buffer, then make leaf-major
+  uint8_t* stage = nullptr;
+  const size_t bytes = (size_t)ncols * m->nl;
+  HIP_TRY(c, dalloc(&stage, std::max<size_t>(bytes, 1)));
+  if (bytes) HIP_TRY(c, hipMemcpy(stage, codes, bytes, hipMemcpyHostToDevice));
+  hipError_t e = launch_transpose_codes(stage, ncols, m->nl, b->d_codes, b->ld, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(stage);
+  HIP_TRY(c, e);
+  for (auto& ev : b->ev) HIP_TRY(c, hipEventCreate(&ev));
+  *out = guard.release();
+  return PCSF_OK;
+}
+
+void pcsf_batch_destroy(pcsf_batch* b) {
+  if (!b) return;
+  dfree(b->d_codes);
+  dfree(b->d_offsets);
+  dfree(b->d_logz);
+  dfree(b->d_colelpr);
+  dfree(b->d_loglik);
+  dfree(b->d_elpr);
+  for (auto& ev : b->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  delete b;
+}
+
+int64_t pcsf_batch_columns(const pcsf_batch* b) { return b ? b->ncols : -1; }
+
+int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int precision,
+                 double* loglik, double* elpr, int save_for_backward) {
+  if (!b || !rho) return PCSF_EINVAL;
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  (void)save_for_backward;
+  if (rho_per_region)
+    return set_err(c, PCSF_EINVAL, "pcsf_forward: per-region rho is not available yet");
+  std::vector<double> r(rho, rho + m->nm);
+  for (double x : r)
+    if (!rho_valid(x))
+      return set_err(c, PCSF_EINVAL,
+                     "CamlPaml.P14n.instantiate_tree: domain violation on variable 0");
+  HIP_TRY(c, hipSetDevice(c->device));
+  float tp = 0.f;
+  int rc = ensure_transitions(m, r, &tp);
+  if (rc) return rc;
+  b->t_pbuild = tp;
+  if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
+  hipError_t e;
+  if (precision == PCSF_FP64) {
+    e = launch_prune_fp64(m, b, c->stream);
+    if (e == hipErrorInvalidValue)
+      return set_err(c, PCSF_EINVAL, "FP64 kernel: tree workspace exceeds 64 KiB of LDS");
+  } else if (precision == PCSF_FP32) {
+    e = launch_prune_mfma(m, b, 0, c->stream);
+  } else {
+    return set_err(c, PCSF_EINVAL, "pcsf_forward: unknown precision");
+  }
+  HIP_TRY(c, e);
+  if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[1], c->stream));
+  HIP_TRY(c, launch_region_reduce(b, c->stream));
+  if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[2], c->stream));
+  const size_t nres = (size_t)b->nreg * m->nm;
+  if (loglik && nres)
+    HIP_TRY(c, hipMemcpyAsync(loglik, b->d_loglik, nres * 8, hipMemcpyDeviceToHost, c->stream));
+  if (elpr && nres)
+    HIP_TRY(c, hipMemcpyAsync(elpr, b->d_elpr, nres * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (c->timing) {
+    HIP_TRY(c, hipEventElapsedTime(&b->t_prune, b->ev[0], b->ev[1]));
+    HIP_TRY(c, hipEventElapsedTime(&b->t_reduce, b->ev[1], b->ev[2]));
+  }
+  return PCSF_OK;
+}
+
+int pcsf_backward(pcsf_batch* b, const double*, double*, double*) {
+  if (!b) return PCSF_EINVAL;
+  return set_err(b->m->ctx, PCSF_EINVAL, "pcsf_backward: adjoint kernel not available yet");
+}
+
+int pcsf_batch_device_results(pcsf_batch* b, const double** d_loglik, const double** d_elpr) {
+  if (!b) return PCSF_EINVAL;
+  if (d_loglik) *d_loglik = b->d_loglik;
+  if (d_elpr) *d_elpr = b->d_elpr;
+  return PCSF_OK;
+}
+
+int pcsf_batch_last_timing(const pcsf_batch* b, double* prune_ms, double* reduce_ms,
+                           double* pbuild_ms) {
+  if (!b) return PCSF_EINVAL;
+  if (prune_ms) *prune_ms = b->t_prune;
+  if (reduce_ms) *reduce_ms = b->t_reduce;
+  if (pbuild_ms) *pbuild_ms = b->t_pbuild;
+  return PCSF_OK;
+}
+
+int pcsf_lik_columns(pcsf_ctx* c, int k, int n, const int32_t* children, const double* P,
+                     const double* prior, const int32_t* leaves, int64_t ncols, double* z,
+                     double* root_post) {
+  if (!c) return PCSF_EINVAL;
+  if (k < 1 || k > 64 || n < 3 || n % 2 == 0 || !children || !P || !prior || !z ||
+      (ncols > 0 && !leaves) || ncols < 0)
+    return set_err(c, PCSF_EINVAL, "CamlPaml.Infer.prepare: bad arguments");
+  const int nl = (n + 1) / 2;
+  std::vector<int32_t> lc(n, -1), rc(n, -1), par(n, -1);
+  for (int i = nl; i < n; ++i) {
+    const int a = children[(i - nl) * 2], b = children[(i - nl) * 2 + 1];
+    if (a < 0 || b < 0 || a >= i || b >= i || par[a] >= 0 || par[b] >= 0 || a == b)
+      return set_err(c, PCSF_EINVAL, "CamlPaml.T: children must precede parents (T order)");
+    lc[i] = a;
+    rc[i] = b;
+    par[a] = par[b] = i;
+  }
+  if ((size_t)n * k * 8 > 65536)
+    return set_err(c, PCSF_EINVAL, "pcsf_lik_columns: workspace exceeds 64 KiB of LDS");
+  if (ncols == 0) return PCSF_OK;
+  HIP_TRY(c, hipSetDevice(c->device));
+  int32_t *d_lc = nullptr, *d_rc = nullptr, *d_leaves = nullptr;
+  double *d_P = nullptr, *d_prior = nullptr, *d_z = nullptr, *d_post = nullptr;
+  auto cleanup = [&]() {
+    dfree(d_lc);
+    dfree(d_rc);
+    dfree(d_leaves);
+    dfree(d_P);
+    dfree(d_prior);
+    dfree(d_z);
+    dfree(d_post);
+  };
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) {
+    if (e == hipSuccess) e = x;
+  };
+  chk(dalloc(&d_lc, (size_t)n));
+  chk(dalloc(&d_rc, (size_t)n));
+  chk(dalloc(&d_leaves, (size_t)ncols * nl));
+  chk(dalloc(&d_P, (size_t)(n - 1) * k * k));
+  chk(dalloc(&d_prior, (size_t)k));
+  chk(dalloc(&d_z, (size_t)ncols));
+  if (root_post) chk(dalloc(&d_post, (size_t)ncols * k));
+  if (e == hipSuccess) {
+    chk(hipMemcpy(d_lc, lc.data(), n * 4, hipMemcpyHostToDevice));
+    chk(hipMemcpy(d_rc, rc.data(), n * 4, hipMemcpyHostToDevice));
+    chk(hipMemcpy(d_leaves, leaves, (size_t)ncols * nl * 4, hipMemcpyHostToDevice));
+    chk(hipMemcpy(d_P, P, (size_t)(n - 1) * k * k * 8, hipMemcpyHostToDevice));
+    chk(hipMemcpy(d_prior, prior, (size_t)k * 8, hipMemcpyHostToDevice));
+  }
+  if (e == hipSuccess)
+    chk(launch_lik_columns(k, n, d_lc, d_rc, d_P, d_prior, d_leaves, ncols, d_z, d_post,
+                           c->stream));
+  if (e == hipSuccess) chk(hipStreamSynchronize(c->stream));
+  if (e == hipSuccess) chk(hipMemcpy(z, d_z, (size_t)ncols * 8, hipMemcpyDeviceToHost));
+  if (e == hipSuccess && root_post)
+    chk(hipMemcpy(root_post, d_post, (size_t)ncols * k * 8, hipMemcpyDeviceToHost));
+  cleanup();
+  HIP_TRY(c, e);
+  return PCSF_OK;
+}
+
+}  // extern "C"

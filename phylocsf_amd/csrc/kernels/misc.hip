@@ -1,0 +1,82 @@
+// misc.hip -- batch packing and per-region reduction kernels.
+//
+//   transpose_codes: the column-major leaf arrays PhyloCSF.pleaves yields
+//     (src/PhyloCSF.ml:219-246; here n_cols x nl bytes) become leaf-major
+//     rows of ld bytes, so a wave reads 32 consecutive columns of one leaf
+//     in one transaction; padding columns are `Marginalize.
+//   region_reduce: lpr = sum ln z and elpr_anc = sum post_root . ln pi over
+//     each region's columns (src/PhyloCSFModel.ml:76-81), FP64, one
+//     workgroup per (region, model), fixed order so results are bitwise
+//     reproducible run to run.
+#include "../internal.hpp"
+
+namespace {
+
+__global__ void transpose_codes_kernel(const uint8_t* __restrict__ src, int64_t ncols, int nl,
+                                       uint8_t* __restrict__ dst, int64_t ld) {
+  // 64 columns x nl leaves per workgroup, staged through LDS
+  __shared__ uint8_t tile[64 * 257];
+  const int64_t c0 = (int64_t)blockIdx.x * 64;
+  for (int l0 = 0; l0 < nl; l0 += 256) {
+    const int lw = min(256, nl - l0);
+    for (int e = threadIdx.x; e < 64 * lw; e += blockDim.x) {
+      const int c = e / lw, l = e - c * lw;
+      const int64_t col = c0 + c;
+      tile[c * 257 + l] = (col < ncols) ? src[col * nl + l0 + l] : (uint8_t)PCSF_MARGINALIZE;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * lw; e += blockDim.x) {
+      const int l = e / 64, c = e - l * 64;
+      const int64_t col = c0 + c;
+      if (col < ld) dst[(size_t)(l0 + l) * ld + col] = tile[c * 257 + l];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void region_reduce_kernel(
+    const double* __restrict__ logz, const double* __restrict__ colelpr,
+    const int64_t* __restrict__ off, int nm, int64_t ld, double* __restrict__ loglik,
+    double* __restrict__ elpr) {
+  __shared__ double s0[256], s1[256];
+  const int r = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
+  const int64_t lo = off[r], hi = off[r + 1];
+  double a = 0.0, b = 0.0;
+  for (int64_t c = lo + tid; c < hi; c += 256) {
+    a += logz[(size_t)q * ld + c];
+    b += colelpr[(size_t)q * ld + c];
+  }
+  s0[tid] = a;
+  s1[tid] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      s0[tid] += s0[tid + w];
+      s1[tid] += s1[tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    loglik[(size_t)r * nm + q] = s0[0];
+    elpr[(size_t)r * nm + q] = s1[0];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uint8_t* dst,
+                                  int64_t ld, hipStream_t s) {
+  const int64_t blocks = (ld + 63) / 64;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(transpose_codes_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src,
+                     ncols, nl, dst, ld);
+  return hipGetLastError();
+}
+
+hipError_t launch_region_reduce(const pcsf_batch* b, hipStream_t s) {
+  if (b->nreg == 0) return hipSuccess;
+  dim3 grid(b->nreg, b->m->nm);
+  hipLaunchKernelGGL(region_reduce_kernel, grid, dim3(256), 0, s, b->d_logz, b->d_colelpr,
+                     b->d_offsets, b->m->nm, b->ld, b->d_loglik, b->d_elpr);
+  return hipGetLastError();
+}

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Tolerances (the scoring path's contract):
+  * FP64 plain kernel: |d score| <= 1e-8 dB, relative log-likelihood <= 1e-12
+    (same arithmetic as the oracle up to summation order and FMA).
+  * FP32 MFMA kernel: |d score| <= 1e-3 dB and relative log-likelihood
+    <= 1e-6 against the FP64 oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import phylocsf_amd as P  # noqa: E402
+from phylocsf_amd import workloads as W  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+TOL_DB_FP32 = 1e-3
+TOL_REL_FP32 = 1e-6
+TOL_DB_FP64 = 1e-8
+TOL_REL_FP64 = 1e-12
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return P.Context(0)
+
+
+def _oracle_model(params_dir, name):
+    prefix = os.path.join(params_dir, name)
+    t = O.newick_tree(open(prefix + ".nh").read())
+    Sc, pc = O.ecm_read(prefix + "_coding.ECM")
+    Sn, pn = O.ecm_read(prefix + "_noncoding.ECM")
+    return O.Model(t, [Sc, Sn], [pc, pn])
+
+
+def _oracle_scores(om, codes, off):
+    out = []
+    for r in range(len(off) - 1):
+        out.append(om.score_fixed(codes[off[r]:off[r + 1]]))
+    return out
+
+
+def _check(res, ref, tol_db, tol_rel, label):
+    for r, s in enumerate(ref):
+        d = abs(res["score"][r] - s["score"])
+        assert d <= tol_db, "%s region %d: |d score| = %g dB" % (label, r, d)
+        for key in ("lpr_c", "lpr_n"):
+            rel = abs(res[key][r] - s[key]) / max(1.0, abs(s[key]))
+            assert rel <= tol_rel, "%s region %d %s rel %g" % (label, r, key, rel)
+        da = abs(res["anc_comp"][r] - s["anc_comp"])
+        assert da <= 10 * tol_db, "%s region %d: |d anc| = %g" % (label, r, da)
+
+
+def test_transition_matrices_match_oracle(ctx, params_dir):
+    for name, rho in (("12flies", 1.0), ("58mammals", 0.7)):
+        _say("P(t) %s rho=%g" % (name, rho))
+        m = P.PhyloCSFModel.from_paramset(ctx, name)
+        prefix = os.path.join(params_dir, name)
+        S, pi = O.ecm_read(prefix + "_coding.ECM")
+        Q = O.build_q(S, pi)
+        Pg = m.transition(0, rho)
+        for br in range(m.tree.n - 1):
+            Po = O.pt(Q, pi, rho * m.tree.branch[br])
+            assert np.allclose(Pg[br], Po, atol=2e-14, rtol=0), br
+
+
+def test_c1_fixed(ctx, params_dir):
+    _say("C1 fixed")
+    m = W.model_for("C1", ctx)
+    wl = W.make("C1", m)
+    assert wl.codes.shape == (100, 12)
+    ref = _oracle_scores(_oracle_model(params_dir, "12flies"), wl.codes, wl.offsets)
+    b = P.Batch(m, wl.codes, wl.offsets)
+    _check(P.score_fixed(b, P.PCSF_FP64), ref, TOL_DB_FP64, TOL_REL_FP64, "C1/FP64")
+    _check(P.score_fixed(b, P.PCSF_FP32), ref, TOL_DB_FP32, TOL_REL_FP32, "C1/FP32")
+
+
+def test_c2_sample_fixed(ctx, params_dir):
+    """58mammals regions with C2's gap/N masks, lengths capped for the oracle."""
+    _say("C2 sample fixed")
+    m = W.model_for("C2", ctx)
+    lengths, rm, rr, kw = W.region_params("C2", 24)
+    lengths = np.minimum(lengths, 257)
+    codes, off = m.simulate(lengths, rm, rr, seed=2, **kw)
+    ref = _oracle_scores(_oracle_model(params_dir, "58mammals"), codes, off)
+    b = P.Batch(m, codes, off)
+    _check(P.score_fixed(b, P.PCSF_FP64), ref, TOL_DB_FP64, TOL_REL_FP64, "C2s/FP64")
+    _check(P.score_fixed(b, P.PCSF_FP32), ref, TOL_DB_FP32, TOL_REL_FP32, "C2s/FP32")
+
+
+def test_other_trees_fixed(ctx, params_dir):
+    """100vertebrates and 120mammals (5 live tiles) and 29mammals, small samples."""
+    for name in ("29mammals", "100vertebrates", "120mammals"):
+        _say("tree %s" % name)
+        m = P.PhyloCSFModel.from_paramset(ctx, name)
+        rng = np.random.default_rng(7)
+        lengths = rng.integers(1, 60, 6)
+        codes, off = m.simulate(lengths, rng.integers(0, 2, 6), np.ones(6), gap_p=0.2, n_p=0.02,
+                                seed=11)
+        ref = _oracle_scores(_oracle_model(params_dir, name), codes, off)
+        b = P.Batch(m, codes, off)
+        _check(P.score_fixed(b, P.PCSF_FP64), ref, TOL_DB_FP64, TOL_REL_FP64, name + "/FP64")
+        _check(P.score_fixed(b, P.PCSF_FP32), ref, TOL_DB_FP32, TOL_REL_FP32, name + "/FP32")
+
+
+def test_examples_fixed(ctx, params_dir, examples_dir):
+    """The reference's example alignments, fixed strategy, frame 1."""
+    from oracle import cli_oracle as CO
+    for pset, fa in (("12flies", "tal-AA.fa"), ("29mammals", "ALDH2.exon5.fa")):
+        _say("example %s" % fa)
+        text = open(os.path.join(examples_dir, fa)).read()
+        om = _oracle_model(params_dir, pset)
+        sp, seqs = CO.input_mfa(text)
+        m = P.PhyloCSFModel.from_paramset(ctx, pset)
+        which = {s: i for i, s in enumerate(sp)}
+        leaf_ord = [which.get(m.tree.labels[i]) for i in range(m.tree.nl)]
+        codes = CO.pleaves(seqs, leaf_ord, 0, len(seqs[0]) - 1)
+        ref = om.score_fixed(codes)
+        b = P.Batch(m, codes, np.array([0, codes.shape[0]]))
+        for prec, tol in ((P.PCSF_FP64, TOL_DB_FP64), (P.PCSF_FP32, TOL_DB_FP32)):
+            s = P.score_fixed(b, prec)
+            assert abs(s["score"][0] - ref["score"]) <= tol
+
+
+def test_phylolik_compat_kats(ctx, params_dir):
+    """lib/CamlPaml/test.ml known answers through pcsf_lik_columns."""
+    _say("PhyloLik compat KATs")
+    import itertools
+    lik = P.PhyloLik(ctx)
+    t = P.newick_tree("(A,(B,C))")
+    sm0 = np.array([[0.8, 0.2], [0.25, 0.75]])
+    sm1 = np.array([[0.9, 0.1], [0.85, 0.15]])
+    pms = np.stack([sm0, sm1, sm1, sm1])
+    prior = np.array([0.6, 0.4])
+    pats = np.array(list(itertools.product([0, 1], repeat=3)))
+    x = lik.prepare(t, pms, prior, pats)
+    for i, lvs in enumerate(pats):
+        a0, a1, a2 = [np.eye(2)[v] for v in lvs]
+        a3 = (sm1 @ a1) * (sm1 @ a2)
+        a4 = (sm0 @ a0) * (sm1 @ a3)
+        z = prior @ a4
+        assert abs(x.likelihood()[i] - z) < 1e-14
+        assert np.allclose(x.node_posterior(4)[i], prior * a4 / z, atol=1e-14)
+    seqs = dict(l.split() for l in open(os.path.join(os.path.dirname(__file__), "golden",
+                                                      "tinytest_sequences.txt")))
+    tt = P.newick_tree("((human:0.1,chimp:0.1):0.1,gorilla:0.2);")
+    Q = (np.ones((4, 4)) - 4 * np.eye(4)) / 3.0
+    pi = np.full(4, 0.25)
+    pms = np.stack([O.pt(Q, pi, b) for b in tt.branch[:-1]])
+    idx = {"A": 0, "C": 1, "G": 2, "T": 3}
+    L = np.array([[idx.get(seqs[s][i], -1) for s in ("human", "chimp", "gorilla")]
+                  for i in range(len(seqs["human"]))])
+    x = lik.prepare(tt, pms, pi, L)
+    assert abs(np.log(x.likelihood()).sum() - (-1574.63623)) < 1e-3
+
+
+def test_edge_cases(ctx, params_dir):
+    """Empty and one-column regions, tile-boundary lengths, all-marginalised columns."""
+    _say("edge cases")
+    m = W.model_for("C2", ctx)
+    lengths = np.array([0, 1, 127, 128, 129, 0, 16, 15, 17, 255, 1])
+    rng = np.random.default_rng(3)
+    codes, off = m.simulate(lengths, rng.integers(0, 2, len(lengths)), np.ones(len(lengths)),
+                            gap_p=0.3, seed=5)
+    codes[off[6]:off[7]] = 255  # a region whose columns are all `Marginalize
+    om = _oracle_model(params_dir, "58mammals")
+    ref = _oracle_scores(om, codes, off)
+    b = P.Batch(m, codes, off)
+    for prec, tdb, trel in ((P.PCSF_FP64, TOL_DB_FP64, TOL_REL_FP64),
+                            (P.PCSF_FP32, TOL_DB_FP32, TOL_REL_FP32)):
+        res = P.score_fixed(b, prec)
+        _check(res, ref, tdb, trel, "edge/%d" % prec)
+        assert res["lpr_c"][0] == 0.0 and res["lpr_c"][5] == 0.0
+        assert abs(res["lpr_c"][6]) < 1e-6 and abs(res["lpr_n"][6]) < 1e-6
+    # a batch with no columns at all
+    b0 = P.Batch(m, np.zeros((0, m.tree.nl), np.uint8), np.array([0, 0]))
+    ll, el = b0.forward(1.0, P.PCSF_FP32)
+    assert ll.shape == (1, 2) and (ll == 0).all()
+
+
+def test_errors(ctx):
+    _say("error paths")
+    m = W.model_for("C1", ctx)
+    b = P.Batch(m, np.zeros((3, 12), np.uint8), np.array([0, 3]))
+    for bad in (0.0, -1.0, float("nan")):
+        with pytest.raises(P.InvalidArgument):
+            b.forward(bad, P.PCSF_FP32)
+    with pytest.raises(P.InvalidArgument):
+        P.Batch(m, np.zeros((3, 12), np.uint8), np.array([0, 2]))
+    with pytest.raises(P.InvalidArgument):
+        m.transition(0, -2.0)
+    # results still valid after the errors
+    ll, _ = b.forward(1.0, P.PCSF_FP32)
+    assert np.isfinite(ll).all()
+
+
+def test_determinism(ctx):
+    _say("determinism")
+    m = W.model_for("C2", ctx)
+    wl = W.make("C2", m, n_regions=50)
+    b = P.Batch(m, wl.codes, wl.offsets)
+    for prec in (P.PCSF_FP32, P.PCSF_FP64):
+        a1, e1 = b.forward(1.0, prec)
+        a2, e2 = b.forward(1.0, prec)
+        assert np.array_equal(a1, a2) and np.array_equal(e1, e2)
+
+
+def test_c2_full_fp32_vs_fp64(ctx, params_dir):
+    """Full C2 workload (10,000 regions): the FP32 MFMA scores against the FP64
+    kernel on every region, and the FP64 kernel against the oracle on a sample."""
+    _say("C2 full: simulate")
+    m = W.model_for("C2", ctx)
+    wl = W.make("C2", m)
+    _say("C2 full: %d columns" % wl.n_cols)
+    b = P.Batch(m, wl.codes, wl.offsets)
+    s64 = P.score_fixed(b, P.PCSF_FP64)
+    s32 = P.score_fixed(b, P.PCSF_FP32)
+    d = np.abs(s32["score"] - s64["score"])
+    _say("C2 full: max |d score| FP32 vs FP64 = %.3g dB (mean %.3g)" % (d.max(), d.mean()))
+    assert d.max() <= TOL_DB_FP32
+    rel = np.abs(s32["lpr_c"] - s64["lpr_c"]) / np.abs(s64["lpr_c"])
+    assert rel.max() <= TOL_REL_FP32
+    om = _oracle_model(params_dir, "58mammals")
+    for r in (0, 4999, 9999):
+        lo, hi = wl.offsets[r], wl.offsets[r + 1]
+        if hi - lo > 600:
+            hi = lo + 600
+            sub = P.Batch(m, wl.codes[lo:hi], np.array([0, hi - lo]))
+            got = P.score_fixed(sub, P.PCSF_FP64)["score"][0]
+        else:
+            got = s64["score"][r]
+        ref = om.score_fixed(wl.codes[lo:hi])["score"]
+        assert abs(got - ref) <= TOL_DB_FP64
