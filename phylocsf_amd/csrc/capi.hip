@@ -346,7 +346,8 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   const size_t pcount = (size_t)nm * m->nbr * 4096;
   HIP_TRY(c, dalloc(&m->d_P64T, pcount));
   HIP_TRY(c, dalloc(&m->d_PT32, pcount));
-  HIP_TRY(c, dalloc(&m->d_Pfrag, pcount));
+  HIP_TRY(c, dalloc(&m->d_Pfrag, (size_t)nm * m->nbr * kTileFloats));
+  HIP_TRY(c, dalloc(&m->d_lnPcc, (size_t)nm * m->nbr * 64));
   std::vector<int32_t> lc32(m->lc.begin(), m->lc.end()), rc32(m->rc.begin(), m->rc.end());
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
@@ -381,6 +382,7 @@ void pcsf_model_destroy(pcsf_model* m) {
   dfree(m->d_P64T);
   dfree(m->d_PT32);
   dfree(m->d_Pfrag);
+  dfree(m->d_lnPcc);
   delete m;
 }
 

@@ -24,6 +24,7 @@ constexpr int kNoBranch = 0xFFFF;
 constexpr int kMaxSlots = 6;
 constexpr int kTileCols = 16;   // columns per wave
 constexpr int kGroupCols = 128; // columns per workgroup (8 waves)
+constexpr int kTileFloats = 4096 + 64;  // P tile staged in LDS: fragments + diagonal
 
 struct pcsf_ctx {
   int device = 0;
@@ -58,8 +59,10 @@ struct pcsf_model {
   // device: transition matrices at the cached tree scales
   std::vector<double> cached_rho;
   double* d_P64T = nullptr;  // nm * nbr * 4096  FP64, P^T (col-major P)
-  float* d_PT32 = nullptr;   // nm * nbr * 4096  FP32, P^T
-  float* d_Pfrag = nullptr;  // nm * nbr * 4096  FP32, (P - I) in MFMA A-fragment order
+  float* d_PT32 = nullptr;   // nm * nbr * 4096  FP32, P^T[c][a] = P[a][c] / P[c][c]
+  float* d_Pfrag = nullptr;  // nm * nbr * 4160  FP32, off-diagonal of P in MFMA A-fragment
+                             //   order, then the 64 diagonal entries P[a][a] - 1
+  double* d_lnPcc = nullptr; // nm * nbr * 64    FP64, ln P[c][c]
   double* d_rho = nullptr;   // nm
   int* d_perr = nullptr;     // error record of the P builder
   double* d_perrval = nullptr;

@@ -9,9 +9,12 @@
 // Outputs (all [model][branch][4096]):
 //   P64T  FP64 P transposed (P64T[b][a] = P[a][b]): coalesced GEMV operand of
 //         the FP64 kernel and its leaf gathers.
-//   PT32  FP32 P transposed: leaf-child gathers of the MFMA kernel.
-//   Pfrag FP32 (P - I) in the A-operand order of v_mfma_f32_16x16x4f32 as
-//         the MFMA kernel consumes it (see prune_mfma.hip).
+//   PT32  FP32 leaf messages of the MFMA kernel: PT32[c][a] = P[a][c] / P[c][c],
+//         with lnPcc[c] = ln P[c][c] in FP64 (the dominant entry of a leaf
+//         message is exactly 1 and its value travels in FP64).
+//   Pfrag FP32 tile of kTileFloats: the off-diagonal of P in the A-operand
+//         order of v_mfma_f32_16x16x4f32 as the MFMA kernel consumes it
+//         (diagonal entries zero), followed by the 64 values P[a][a] - 1.
 #include "../internal.hpp"
 
 namespace {
@@ -22,8 +25,8 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
     const double* __restrict__ V, const double* __restrict__ Vinv,
     const double* __restrict__ lam, const double* __restrict__ branch,
     const double* __restrict__ rho, int nbr, double* __restrict__ P64T,
-    float* __restrict__ PT32, float* __restrict__ Pfrag, int* __restrict__ perr,
-    double* __restrict__ perrval) {
+    float* __restrict__ PT32, float* __restrict__ Pfrag, double* __restrict__ lnPcc,
+    int* __restrict__ perr, double* __restrict__ perrval) {
   __shared__ double W[64 * 64];  // diag(exp(lam t)) V^-1, then P
   __shared__ double Vs[64 * 64];
   const int br = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
@@ -89,8 +92,10 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
     const int b = e >> 6, a = e & 63;  // transposed: [b][a] = P[a][b]
     const double p = Vs[a * 64 + b];
     P64T[base + e] = p;
-    PT32[base + e] = (float)p;
+    PT32[base + e] = (float)(p / Vs[b * 64 + b]);
   }
+  if (tid < 64) lnPcc[((size_t)q * nbr + br) * 64 + tid] = log(Vs[tid * 64 + tid]);
+  const size_t fbase = ((size_t)q * nbr + br) * kTileFloats;
   // fragment order of v_mfma_f32_16x16x4f32's A operand with the K index
   // permuted to match the accumulator layout (prune_mfma.hip):
   //   e = ((mo*4 + mb)*64 + lane)*4 + v
@@ -99,9 +104,9 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
     const int v = e & 3, lane = (e >> 2) & 63, mb = (e >> 8) & 3, mo = e >> 10;
     const int row = 16 * mo + (lane & 15);
     const int col = 16 * mb + 4 * (lane >> 4) + v;
-    const double d = Vs[row * 64 + col] - (row == col ? 1.0 : 0.0);
-    Pfrag[base + e] = (float)d;
+    Pfrag[fbase + e] = (row == col) ? 0.0f : (float)Vs[row * 64 + col];
   }
+  if (tid < 64) Pfrag[fbase + 4096 + tid] = (float)(Vs[tid * 64 + tid] - 1.0);
 }
 
 }  // namespace
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
 hipError_t launch_pbuild(const pcsf_model* m, const double* d_rho, hipStream_t s) {
   dim3 grid(m->nbr, m->nm);
   hipLaunchKernelGGL(pbuild_kernel, grid, dim3(256), 0, s, m->d_V, m->d_Vinv, m->d_lam,
-                     m->d_branch, d_rho, m->nbr, m->d_P64T, m->d_PT32, m->d_Pfrag, m->d_perr,
-                     m->d_perrval);
+                     m->d_branch, d_rho, m->nbr, m->d_P64T, m->d_PT32, m->d_Pfrag, m->d_lnPcc,
+                     m->d_perr, m->d_perrval);
   return hipGetLastError();
 }

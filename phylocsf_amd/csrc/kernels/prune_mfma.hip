@@ -16,13 +16,20 @@
 //    as the B operand of the next GEMM by permuting the K index, so partials
 //    move between GEMMs without any shuffle; the P tile is stored in the
 //    matching permuted order (pbuild.hip) and read with ds_read_b128.
-//  * The accumulator starts at alpha and the A operand is P - I, so
-//    m = alpha + (P - I) alpha: for short branches FP32 rounding acts on the
-//    small correction only.
-//  * Leaf children are gathers of a column of P (or 1 for `Marginalize);
-//    cherries need no GEMM.  After every product each column is rescaled by
-//    the power of two of its maximum (exact) and the exponents are summed
-//    per column; the root dot and the logarithms are FP64.
+//  * FP32 MFMA accumulation rounds with a bias toward zero (measured on
+//    MI355X: about -2e-6 per column and model when the accumulator carries
+//    alpha).  So the MFMA only accumulates the off-diagonal mass of P from a
+//    zero accumulator, and m = alpha + fma(P_aa - 1, alpha, offdiag) is
+//    finished in round-to-nearest VALU arithmetic.
+//  * Leaf children are gathers of a column of P normalised by its diagonal
+//    entry (so the dominant entry is exactly 1; ln P[c][c] is summed per
+//    column in FP64), or 1 for `Marginalize; cherries need no GEMM.  A lane mask per live tile marks columns whose
+//    partial is a constant vector (a fully marginalised subtree); P c1 = c1
+//    exactly, so those columns skip the GEMM update instead of picking up
+//    the rounding of the FP32 row sums of P - I.  After every product each
+//    column is rescaled by the power of two of its maximum (exact) and the
+//    exponents are summed per column; the root dot and the logarithms are
+//    FP64.
 //  * P tiles are double-buffered in LDS: while a GEMM runs, every thread
 //    prefetches its share of the next GEMM's tile from L2.
 //  * Workgroups are mapped to models by XCD (blockIdx % 8), so each XCD's
@@ -43,6 +50,7 @@ struct MfmaParams {
   const float* PT32;
   const double* pi;
   const double* lnpi;
+  const double* lnPcc;
   double* logz;
   double* colelpr;
 };
@@ -76,17 +84,21 @@ struct Tile {
   f32x4 r[4];
 };
 
-__device__ __forceinline__ void gather(Tile& g, const MfmaParams& p, int q, int leaf,
-                                       int64_t col, int grp) {
+__device__ __forceinline__ bool gather(Tile& g, const MfmaParams& p, int q, int leaf,
+                                       int64_t col, int grp, double& ls) {
   const int code = p.codes[(size_t)leaf * p.ld + col];
   const bool marg = code == PCSF_MARGINALIZE;
   const int c = marg ? 0 : code;
-  const float* src = p.PT32 + (((size_t)q * p.nbr + leaf) * 64 + c) * 64 + 4 * grp;
+  const size_t bc = ((size_t)q * p.nbr + leaf) * 64 + c;
+  const float* src = p.PT32 + bc * 64 + 4 * grp;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) {
     const f32x4 v = *reinterpret_cast<const f32x4*>(src + 16 * mb);
     g.r[mb] = marg ? (f32x4)(1.0f) : v;
   }
+  const double l = p.lnPcc[bc];
+  ls += marg ? 0.0 : l;
+  return marg;
 }
 
 __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
@@ -110,27 +122,34 @@ __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
 template <int NS, int WPG>
 __global__ __launch_bounds__(64 * WPG) void prune_mfma_kernel(MfmaParams p) {
   constexpr int NT = 64 * WPG;          // threads
-  constexpr int PER = 4096 / NT / 4;    // float4 of a P tile per thread
+  constexpr int PER = 4096 / NT / 4;    // float4 of the fragments per thread
   constexpr int GC = 16 * WPG;          // columns per workgroup
-  __shared__ __attribute__((aligned(16))) float Pbuf[2][4096];
+  static_assert(NT >= 16, "the diagonal (16 float4) is copied by the first 16 threads");
+  __shared__ __attribute__((aligned(16))) float Pbuf[2][kTileFloats];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int grp = lane >> 4, lcol = lane & 15;
 
   Tile S[NS];
+  uint64_t K[NS];  // lanes whose tile column is a constant vector
 #pragma unroll
-  for (int d = 0; d < NS; ++d)
+  for (int d = 0; d < NS; ++d) {
+    K[d] = 0;
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) S[d].r[mb] = (f32x4)(0.0f);
+  }
 
   Task cur;
   if (!task_at(p, 0, &cur)) return;
   int kg = 0;
   if (p.first_gemm_br != kNoBranch) {
-    const float* src = p.Pfrag + ((size_t)cur.q * p.nbr + p.first_gemm_br) * 4096;
+    const float* src = p.Pfrag + ((size_t)cur.q * p.nbr + p.first_gemm_br) * kTileFloats;
 #pragma unroll
     for (int r = 0; r < PER; ++r)
       *reinterpret_cast<f32x4*>(&Pbuf[0][(tid + NT * r) * 4]) =
           *reinterpret_cast<const f32x4*>(src + (tid + NT * r) * 4);
+    if (tid < 16)
+      *reinterpret_cast<f32x4*>(&Pbuf[0][4096 + tid * 4]) =
+          *reinterpret_cast<const f32x4*>(src + 4096 + tid * 4);
   }
 
   for (int64_t j = 0;; ++j) {
@@ -139,6 +158,7 @@ __global__ __launch_bounds__(64 * WPG) void prune_mfma_kernel(MfmaParams p) {
     const int q = cur.q;
     const int64_t col = cur.g * GC + wave * 16 + lcol;
     int E = 0;
+    double LS = 0.0;  // sum of ln P[c][c] of the gathered leaves
     for (int pc = 0;; ++pc) {
       const int w0 = __builtin_amdgcn_readfirstlane(p.prog[2 * pc]);
       const int w1 = __builtin_amdgcn_readfirstlane(p.prog[2 * pc + 1]);
@@ -146,32 +166,36 @@ __global__ __launch_bounds__(64 * WPG) void prune_mfma_kernel(MfmaParams p) {
       const int a0 = w1 & 0xffff, a1 = (w1 >> 16) & 0xffff;
       if (op == OP_END) break;
 
-      auto do_cherry = [&](Tile& s) {
+      auto do_cherry = [&](Tile& s, uint64_t& k) {
         Tile g;
-        gather(s, p, q, a0, col, grp);
-        gather(g, p, q, a1, col, grp);
+        const bool m0 = gather(s, p, q, a0, col, grp, LS);
+        const bool m1 = gather(g, p, q, a1, col, grp, LS);
+        k = __ballot(m0 && m1);
         mul_rescale(s, g, E);
       };
-      auto do_leafmul = [&](Tile& s) {
+      auto do_leafmul = [&](Tile& s, uint64_t& k) {
         Tile g;
-        gather(g, p, q, a0, col, grp);
+        const bool m0 = gather(g, p, q, a0, col, grp, LS);
+        k &= __ballot(m0);
         mul_rescale(s, g, E);
       };
-      auto do_gemm = [&](Tile& s) {
+      auto do_gemm = [&](Tile& s, uint64_t k) {
         __syncthreads();  // Pbuf[kg&1] complete; Pbuf[(kg&1)^1] no longer read
         const float* nsrc = nullptr;
-        if (a1 != kNoBranch) nsrc = p.Pfrag + ((size_t)q * p.nbr + a1) * 4096;
-        else if (has_next) nsrc = p.Pfrag + ((size_t)nxt.q * p.nbr + p.first_gemm_br) * 4096;
-        f32x4 pf[PER];
+        if (a1 != kNoBranch) nsrc = p.Pfrag + ((size_t)q * p.nbr + a1) * kTileFloats;
+        else if (has_next)
+          nsrc = p.Pfrag + ((size_t)nxt.q * p.nbr + p.first_gemm_br) * kTileFloats;
+        f32x4 pf[PER + 1];
         if (nsrc) {
 #pragma unroll
           for (int r = 0; r < PER; ++r)
             pf[r] = *reinterpret_cast<const f32x4*>(nsrc + (tid + NT * r) * 4);
+          if (tid < 16) pf[PER] = *reinterpret_cast<const f32x4*>(nsrc + 4096 + tid * 4);
         }
         const float* Ab = Pbuf[kg & 1];
         f32x4 acc[4];
 #pragma unroll
-        for (int mo = 0; mo < 4; ++mo) acc[mo] = s.r[mo];
+        for (int mo = 0; mo < 4; ++mo) acc[mo] = (f32x4)(0.0f);
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
           f32x4 A[4];
@@ -186,13 +210,23 @@ __global__ __launch_bounds__(64 * WPG) void prune_mfma_kernel(MfmaParams p) {
               acc[mo] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mo][v], bv, acc[mo], 0, 0, 0);
           }
         }
+        // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, in round-to-nearest VALU
 #pragma unroll
-        for (int mo = 0; mo < 4; ++mo) s.r[mo] = acc[mo];
+        for (int mo = 0; mo < 4; ++mo) {
+          const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            acc[mo][v] = s.r[mo][v] + fmaf(dd[v], s.r[mo][v], acc[mo][v]);
+        }
+        const bool cst = (k >> lane) & 1;
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo) s.r[mo] = cst ? s.r[mo] : acc[mo];
         if (nsrc) {
           float* dst = Pbuf[(kg & 1) ^ 1];
 #pragma unroll
           for (int r = 0; r < PER; ++r)
             *reinterpret_cast<f32x4*>(&dst[(tid + NT * r) * 4]) = pf[r];
+          if (tid < 16) *reinterpret_cast<f32x4*>(&dst[4096 + tid * 4]) = pf[PER];
         }
         ++kg;
       };
@@ -214,23 +248,26 @@ __global__ __launch_bounds__(64 * WPG) void prune_mfma_kernel(MfmaParams p) {
         z += __shfl_xor(z, 32);
         el += __shfl_xor(el, 32);
         if (grp == 0) {
-          p.logz[(size_t)q * p.ld + col] = log(z) + (double)E * 0.69314718055994530942;
+          p.logz[(size_t)q * p.ld + col] = log(z) + (double)E * 0.69314718055994530942 + LS;
           p.colelpr[(size_t)q * p.ld + col] = (z == 0.0) ? 0.0 : el / z;
         }
       };
 
-#define PCSF_CASES(D)                               \
-  case OP_CHERRY * 8 + D:                           \
-    if constexpr (D < NS) do_cherry(S[D]);          \
-    break;                                          \
-  case OP_GEMM * 8 + D:                             \
-    if constexpr (D < NS) do_gemm(S[D]);            \
-    break;                                          \
-  case OP_LEAFMUL * 8 + D:                          \
-    if constexpr (D < NS) do_leafmul(S[D]);         \
-    break;                                          \
-  case OP_MULPOP * 8 + D:                           \
-    if constexpr (D + 1 < NS) mul_rescale(S[D], S[D + 1], E); \
+#define PCSF_CASES(D)                                  \
+  case OP_CHERRY * 8 + D:                              \
+    if constexpr (D < NS) do_cherry(S[D], K[D]);       \
+    break;                                             \
+  case OP_GEMM * 8 + D:                                \
+    if constexpr (D < NS) do_gemm(S[D], K[D]);         \
+    break;                                             \
+  case OP_LEAFMUL * 8 + D:                             \
+    if constexpr (D < NS) do_leafmul(S[D], K[D]);      \
+    break;                                             \
+  case OP_MULPOP * 8 + D:                              \
+    if constexpr (D + 1 < NS) {                        \
+      K[D] &= K[D + 1];                                \
+      mul_rescale(S[D], S[D + 1], E);                  \
+    }                                                  \
     break;
 
       switch (op * 8 + d) {
@@ -290,6 +327,7 @@ hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int grid,
   prm.PT32 = m->d_PT32;
   prm.pi = m->d_pi;
   prm.lnpi = m->d_lnpi;
+  prm.lnPcc = m->d_lnPcc;
   prm.logz = b->d_logz;
   prm.colelpr = b->d_colelpr;
   if (prm.ngroups == 0) return hipSuccess;
