@@ -104,7 +104,12 @@ def main():
     nl = model.tree.nl
     ncols = wl.n_cols
 
-    # ---- batched FP32 MFMA path (the measured value)
+    # ---- batched FP32 MFMA path (the measured value); the autotuner picks the
+    # launch shape before the timed region
+    batch.forward(1.0, P.PCSF_FP32)
+    chosen, shape_ms = batch.autotune(reps=2)
+    log("autotune: " + ", ".join("v%d %.2f ms" % (i, t) for i, t in enumerate(shape_ms)) +
+        " -> v%d" % chosen)
     for _ in range(args.warmup):
         batch.forward(1.0, P.PCSF_FP32)
     ctx.synchronize()
@@ -142,6 +147,8 @@ def main():
     roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP32_TFLOPS, traffic=None,
                     kernel="prune_mfma_kernel", kernel_ms=float(np.mean(kernel_ms)),
+                    launch_shape=int(chosen),
+                    autotune_ms=[round(float(t), 3) for t in shape_ms],
                     flops_per_column=flops_per_col)
 
     cpu = None

@@ -123,6 +123,16 @@ int pcsf_forward(pcsf_batch *b, const double *rho, int rho_per_region, int preci
 int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
                   double *grad_branch);
 
+/* Autotuner of the fused FP32 kernel: times every launch shape (waves per
+ * workgroup sharing one LDS P tile x register cap) on this batch, `reps`
+ * launches each, keeps the fastest on the batch's model for later forwards
+ * and returns the per-shape milliseconds in ms_out[PCSF_N_VARIANTS].
+ * No counterpart in the reference (a tuning step of this implementation). */
+#define PCSF_N_VARIANTS 5
+int pcsf_autotune(pcsf_batch *b, int reps, double *ms_out, int *chosen);
+/* Select a launch shape by index (0..PCSF_N_VARIANTS-1) without timing. */
+int pcsf_set_variant(pcsf_model *m, int variant);
+
 /* Device pointers of the last forward's per-region results (FP64,
  * n_regions*n_models each) and the kernel times when timing is enabled. */
 int pcsf_batch_device_results(pcsf_batch *b, const double **d_loglik, const double **d_elpr);

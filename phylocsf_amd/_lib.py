@@ -26,7 +26,7 @@ EXPORTED = [
     "pcsf_model_create", "pcsf_model_destroy", "pcsf_model_info", "pcsf_model_transition",
     "pcsf_batch_create", "pcsf_batch_destroy", "pcsf_batch_columns", "pcsf_forward",
     "pcsf_backward", "pcsf_batch_device_results", "pcsf_batch_last_timing",
-    "pcsf_lik_columns", "pcsf_simulate",
+    "pcsf_lik_columns", "pcsf_simulate", "pcsf_autotune", "pcsf_set_variant",
 ]
 
 _lib = None
@@ -101,5 +101,7 @@ def lib():
     L.pcsf_lik_columns.argtypes = [vp, C.c_int, C.c_int, i32p, dp, dp, i32p, C.c_int64, dp, dp]
     L.pcsf_simulate.argtypes = [vp, C.c_int32, i64p, i32p, dp, C.c_int, C.c_double,
                                 C.c_double, C.c_double, C.c_uint64, C.c_int, u8p]
+    L.pcsf_autotune.argtypes = [vp, C.c_int, dp, ip]
+    L.pcsf_set_variant.argtypes = [vp, C.c_int]
     _lib = L
     return L

@@ -162,6 +162,9 @@ class PhyloCSFModel:
         return cls(ctx, tree, [import_parameters(prefix + "_coding.ECM"),
                                import_parameters(prefix + "_noncoding.ECM")])
 
+    def set_variant(self, v):
+        self.ctx.check(lib().pcsf_set_variant(self.h, int(v)))
+
     def info(self):
         n, nl, nm, sl = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self.ctx.check(lib().pcsf_model_info(self.h, C.byref(n), C.byref(nl), C.byref(nm),
@@ -241,6 +244,14 @@ class Batch:
         self.model.ctx.check(lib().pcsf_forward(self.h, _dp(r), 1 if per_region else 0,
                                                 precision, _dp(ll), _dp(el), 0))
         return ll, el
+
+    def autotune(self, reps=3):
+        """Time every launch shape of the fused kernel on this batch and keep the
+        fastest for this model: -> (chosen index, [ms per shape])."""
+        ms = np.zeros(5)
+        ch = C.c_int(0)
+        self.model.ctx.check(lib().pcsf_autotune(self.h, reps, _dp(ms), C.byref(ch)))
+        return ch.value, ms
 
     def timing(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

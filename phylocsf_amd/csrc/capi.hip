@@ -495,7 +495,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
     if (e == hipErrorInvalidValue)
       return set_err(c, PCSF_EINVAL, "FP64 kernel: tree workspace exceeds 64 KiB of LDS");
   } else if (precision == PCSF_FP32) {
-    e = launch_prune_mfma(m, b, 0, c->stream);
+    e = launch_prune_mfma(m, b, m->mfma_variant, c->stream);
   } else {
     return set_err(c, PCSF_EINVAL, "pcsf_forward: unknown precision");
   }
@@ -513,6 +513,53 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
     HIP_TRY(c, hipEventElapsedTime(&b->t_prune, b->ev[0], b->ev[1]));
     HIP_TRY(c, hipEventElapsedTime(&b->t_reduce, b->ev[1], b->ev[2]));
   }
+  return PCSF_OK;
+}
+
+int pcsf_autotune(pcsf_batch* b, int reps, double* ms_out, int* chosen) {
+  if (!b || reps < 1) return PCSF_EINVAL;
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (m->cached_rho.empty()) {
+    std::vector<double> one(m->nm, 1.0);
+    int rc = ensure_transitions(m, one, nullptr);
+    if (rc) return rc;
+  }
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  double best = 1e300;
+  int best_v = m->mfma_variant;
+  hipError_t err = hipSuccess;
+  for (int v = 0; v < kNumMfmaVariants && err == hipSuccess; ++v) {
+    err = launch_prune_mfma(m, b, v, c->stream);  // warm-up
+    if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch_prune_mfma(m, b, v, c->stream);
+    if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    const double per = ms / reps;
+    if (ms_out) ms_out[v] = per;
+    if (err == hipSuccess && per < best) {
+      best = per;
+      best_v = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIP_TRY(c, err);
+  m->mfma_variant = best_v;
+  if (chosen) *chosen = best_v;
+  return PCSF_OK;
+}
+
+int pcsf_set_variant(pcsf_model* m, int variant) {
+  if (!m) return PCSF_EINVAL;
+  if (variant < 0 || variant >= kNumMfmaVariants)
+    return set_err(m->ctx, PCSF_EINVAL, "pcsf_set_variant: no such launch variant");
+  m->mfma_variant = variant;
   return PCSF_OK;
 }
 

@@ -45,6 +45,7 @@ struct pcsf_model {
   std::vector<int32_t> prog;  // pairs
   int slots = 0;
   int n_gemm = 0;
+  int mfma_variant = 1;  // launch shape of the fused kernel (pcsf_autotune)
   int first_gemm_br = kNoBranch;
   // device: model constants
   double* d_V = nullptr;     // nm * 4096
@@ -87,7 +88,15 @@ hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uin
                                   int64_t ld, hipStream_t s);
 hipError_t launch_pbuild(const pcsf_model* m, const double* d_rho, hipStream_t s);
 hipError_t launch_prune_fp64(const pcsf_model* m, const pcsf_batch* b, hipStream_t s);
-hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int grid,
+// Launch shapes of the fused kernel that the autotuner chooses among.
+struct MfmaVariant {
+  int wpg;   // waves per workgroup (16 columns each) sharing one LDS P tile
+  int minw;  // minimum waves per SIMD the compiler must allow (register cap)
+  const char* what;
+};
+constexpr int kNumMfmaVariants = 5;
+extern const MfmaVariant kMfmaVariants[kNumMfmaVariants];
+hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int variant,
                              hipStream_t s);
 hipError_t launch_region_reduce(const pcsf_batch* b, hipStream_t s);
 hipError_t launch_lik_columns(int k, int n, const int32_t* d_lc, const int32_t* d_rc,

@@ -119,8 +119,8 @@ __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
   E += e;
 }
 
-template <int NS, int WPG>
-__global__ __launch_bounds__(64 * WPG) void prune_mfma_kernel(MfmaParams p) {
+template <int NS, int WPG, int MINW>
+__global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p) {
   constexpr int NT = 64 * WPG;          // threads
   constexpr int PER = 4096 / NT / 4;    // float4 of the fragments per thread
   constexpr int GC = 16 * WPG;          // columns per workgroup
@@ -290,34 +290,49 @@ __global__ __launch_bounds__(64 * WPG) void prune_mfma_kernel(MfmaParams p) {
   }
 }
 
-template <int NS, int WPG>
-hipError_t launch_ns(const MfmaParams& prm, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((prune_mfma_kernel<NS, WPG>), dim3(grid), dim3(64 * WPG), 0, s, prm);
+template <int NS, int WPG, int MINW>
+hipError_t launch_ns(const MfmaParams& prm, int grid_per_cu, int num_cus, hipStream_t s) {
+  auto kern = prune_mfma_kernel<NS, WPG, MINW>;
+  if (grid_per_cu <= 0) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WPG, 0);
+    if (e != hipSuccess) return e;
+    grid_per_cu = per_cu > 0 ? per_cu : 1;
+  }
+  int grid = (grid_per_cu * num_cus + 7) / 8 * 8;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * WPG), 0, s, prm);
   return hipGetLastError();
 }
 
-template <int WPG>
-hipError_t launch_wpg(const MfmaParams& prm, int slots, int grid, hipStream_t s) {
+template <int WPG, int MINW>
+hipError_t launch_v(const MfmaParams& prm, int slots, int gpc, int cus, hipStream_t s) {
   switch (slots) {
     case 1:
-    case 2: return launch_ns<2, WPG>(prm, grid, s);
-    case 3: return launch_ns<3, WPG>(prm, grid, s);
-    case 4: return launch_ns<4, WPG>(prm, grid, s);
-    case 5: return launch_ns<5, WPG>(prm, grid, s);
-    case 6: return launch_ns<6, WPG>(prm, grid, s);
+    case 2: return launch_ns<2, WPG, MINW>(prm, gpc, cus, s);
+    case 3: return launch_ns<3, WPG, MINW>(prm, gpc, cus, s);
+    case 4: return launch_ns<4, WPG, MINW>(prm, gpc, cus, s);
+    case 5: return launch_ns<5, WPG, MINW>(prm, gpc, cus, s);
+    case 6: return launch_ns<6, WPG, MINW>(prm, gpc, cus, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 }  // namespace
 
-hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int grid,
+const MfmaVariant kMfmaVariants[kNumMfmaVariants] = {
+    {8, 1, "8 waves/WG, >=1 wave/SIMD"}, {4, 2, "4 waves/WG, >=2 waves/SIMD"},
+    {4, 3, "4 waves/WG, >=3 waves/SIMD"}, {8, 3, "8 waves/WG, >=3 waves/SIMD"},
+    {2, 2, "2 waves/WG, >=2 waves/SIMD"},
+};
+
+hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int variant,
                              hipStream_t s) {
-  constexpr int WPG = kGroupCols / 16;
+  if (variant < 0 || variant >= kNumMfmaVariants) return hipErrorInvalidValue;
+  const int wpg = kMfmaVariants[variant].wpg;
   MfmaParams prm;
   prm.codes = b->d_codes;
   prm.ld = b->ld;
-  prm.ngroups = b->ld / kGroupCols;
+  prm.ngroups = b->ld / (16 * wpg);
   prm.nl = m->nl;
   prm.nm = m->nm;
   prm.nbr = m->nbr;
@@ -331,7 +346,13 @@ hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int grid,
   prm.logz = b->d_logz;
   prm.colelpr = b->d_colelpr;
   if (prm.ngroups == 0) return hipSuccess;
-  if (grid <= 0) grid = m->ctx->num_cus;
-  grid = (grid + 7) / 8 * 8;
-  return launch_wpg<WPG>(prm, m->slots, grid, s);
+  const int cus = m->ctx->num_cus;
+  switch (variant) {
+    case 0: return launch_v<8, 1>(prm, m->slots, 0, cus, s);
+    case 1: return launch_v<4, 2>(prm, m->slots, 0, cus, s);
+    case 2: return launch_v<4, 3>(prm, m->slots, 0, cus, s);
+    case 3: return launch_v<8, 3>(prm, m->slots, 0, cus, s);
+    case 4: return launch_v<2, 2>(prm, m->slots, 0, cus, s);
+    default: return hipErrorInvalidValue;
+  }
 }
