@@ -73,6 +73,16 @@ int pcsf_newick_tree(const char *text, const char *keep_csv, int max_nodes, int 
                      int32_t *children, double *branch, char *labels_buf, int labels_len,
                      char *err, int errlen);
 
+/* The op program the fused kernel runs for a tree (host only): a post-order
+ * walk that keeps the fewest partial tiles live, four int32 words per op
+ * (opcode | depth << 8 | prefetched << 16, leaf/branch args, prefetch leaf or
+ * table index, 0).  with_tables = 1 gives the fixed-scale program in which
+ * messages above cherries come from lookup tables.  No counterpart in the
+ * reference (it is this implementation's schedule of PhyloLik.ml:78-90);
+ * exposed so the schedule can be checked without a GPU. */
+int pcsf_program(int n_nodes, const int32_t *children, int with_tables, int32_t *ops, int cap,
+                 int *n_ops, int *slots);
+
 /* ---- model: tree + rate models --------------------------------------- */
 
 /* PhyloCSFModel.make generalised to n_models ECMs on one tree
@@ -123,12 +133,27 @@ int pcsf_forward(pcsf_batch *b, const double *rho, int rho_per_region, int preci
 int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
                   double *grad_branch);
 
+/* Per-region scores, PhyloCSFModel.score (src/PhyloCSFModel.ml:138-146) for
+ * a batch of a two-model (coding, noncoding) model:
+ *   PCSF_STRATEGY_FIXED  llr_FixedLik 1.0 (src/PhyloCSFModel.ml:122-128)
+ *   PCSF_STRATEGY_MLE    llr_MaxLik ~init:1 (src/PhyloCSFModel.ml:84-99,
+ *                        130-136): Fit.find_init then GSL Brent per region and
+ *                        model, all regions advancing in lock step on the GPU
+ * score / anc_comp in decibans (n_regions); rho, lpr, elpr per region and
+ * model (n_regions*2, coding first); status per region (PCSF_OK, or
+ * PCSF_EFAIL where the reference raises inside the maximiser);
+ * n_evals = likelihood evaluations per region.  Any output may be NULL. */
+#define PCSF_STRATEGY_FIXED 0
+#define PCSF_STRATEGY_MLE 1
+int pcsf_score(pcsf_batch *b, int strategy, int precision, double *score, double *anc_comp,
+               double *rho, double *lpr, double *elpr, int32_t *status, int32_t *n_evals);
+
 /* Autotuner of the fused FP32 kernel: times every launch shape (waves per
  * workgroup sharing one LDS P tile x register cap) on this batch, `reps`
  * launches each, keeps the fastest on the batch's model for later forwards
  * and returns the per-shape milliseconds in ms_out[PCSF_N_VARIANTS].
  * No counterpart in the reference (a tuning step of this implementation). */
-#define PCSF_N_VARIANTS 5
+#define PCSF_N_VARIANTS 6
 int pcsf_autotune(pcsf_batch *b, int reps, double *ms_out, int *chosen);
 /* Select a launch shape by index (0..PCSF_N_VARIANTS-1) without timing. */
 int pcsf_set_variant(pcsf_model *m, int variant);

@@ -7,8 +7,8 @@ falls back to a CPU implementation.
 from ._lib import (PCSF_FP32, PCSF_FP64, PCSF_MARGINALIZE, Failure, InvalidArgument,  # noqa
                    PcsfError)
 from .api import (DB, Batch, Context, PhyloCSFModel, PhyloLik, Tree, import_parameters,  # noqa
-                  newick_tree, paramset_prefix, score_fixed)
+                  newick_tree, paramset_prefix, score, score_fixed)
 
 __all__ = ["Context", "PhyloCSFModel", "Batch", "PhyloLik", "Tree", "import_parameters",
-           "newick_tree", "paramset_prefix", "score_fixed", "DB", "PCSF_FP32", "PCSF_FP64",
+           "newick_tree", "paramset_prefix", "score", "score_fixed", "DB", "PCSF_FP32", "PCSF_FP64",
            "PCSF_MARGINALIZE", "PcsfError", "InvalidArgument", "Failure"]

@@ -26,8 +26,11 @@ EXPORTED = [
     "pcsf_model_create", "pcsf_model_destroy", "pcsf_model_info", "pcsf_model_transition",
     "pcsf_batch_create", "pcsf_batch_destroy", "pcsf_batch_columns", "pcsf_forward",
     "pcsf_backward", "pcsf_batch_device_results", "pcsf_batch_last_timing",
-    "pcsf_lik_columns", "pcsf_simulate", "pcsf_autotune", "pcsf_set_variant",
+    "pcsf_lik_columns", "pcsf_simulate", "pcsf_autotune", "pcsf_set_variant", "pcsf_score",
+    "pcsf_program",
 ]
+PCSF_STRATEGY_FIXED = 0
+PCSF_STRATEGY_MLE = 1
 
 _lib = None
 
@@ -103,5 +106,7 @@ def lib():
                                 C.c_double, C.c_double, C.c_uint64, C.c_int, u8p]
     L.pcsf_autotune.argtypes = [vp, C.c_int, dp, ip]
     L.pcsf_set_variant.argtypes = [vp, C.c_int]
+    L.pcsf_score.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, dp, dp, i32p, i32p]
+    L.pcsf_program.argtypes = [C.c_int, i32p, C.c_int, i32p, C.c_int, ip, ip]
     _lib = L
     return L

@@ -18,7 +18,8 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from ._lib import (PCSF_FP32, PCSF_FP64, PCSF_MARGINALIZE, InvalidArgument, lib, raise_for)
+from ._lib import (PCSF_FP32, PCSF_FP64, PCSF_MARGINALIZE, PCSF_STRATEGY_FIXED,
+                   PCSF_STRATEGY_MLE, InvalidArgument, lib, raise_for)
 
 DB = 10.0 / np.log(10.0)  # PhyloCSFModel.db (src/PhyloCSFModel.ml:113)
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -103,6 +104,23 @@ def paramset_prefix(paramset: str) -> str:
                                "directory of the source or executable distribution")
         return os.path.join(base, "PhyloCSF_Parameters", paramset)
     return os.path.join(PARAMS_DIR, paramset)
+
+
+def program(tree: Tree, with_tables=False):
+    """The fused kernel's op program for a tree: (ops [n_ops, 4] int32, slots)."""
+    ch = np.ascontiguousarray(tree.children, dtype=np.int32)
+    n = C.c_int(0)
+    sl = C.c_int(0)
+    rc = lib().pcsf_program(tree.n, _i32(ch), 1 if with_tables else 0, None, 0, C.byref(n),
+                            C.byref(sl))
+    if rc:
+        raise_for(rc, "pcsf_program: invalid tree")
+    ops = np.zeros(n.value * 4, dtype=np.int32)
+    rc = lib().pcsf_program(tree.n, _i32(ch), 1 if with_tables else 0, _i32(ops), ops.size,
+                            C.byref(n), C.byref(sl))
+    if rc:
+        raise_for(rc, "pcsf_program failed")
+    return ops.reshape(-1, 4), sl.value
 
 
 class Context:
@@ -248,7 +266,7 @@ class Batch:
     def autotune(self, reps=3):
         """Time every launch shape of the fused kernel on this batch and keep the
         fastest for this model: -> (chosen index, [ms per shape])."""
-        ms = np.zeros(5)
+        ms = np.zeros(6)
         ch = C.c_int(0)
         self.model.ctx.check(lib().pcsf_autotune(self.h, reps, _dp(ms), C.byref(ch)))
         return ch.value, ms
@@ -269,6 +287,24 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def score(batch: Batch, strategy="fixed", precision=PCSF_FP32):
+    """PhyloCSFModel.score for every region of a batch (src/PhyloCSFModel.ml:138-146).
+
+    strategy "fixed" (llr_FixedLik 1.0) or "mle" (llr_MaxLik: Fit.find_init + GSL
+    Brent per region and model).  -> dict of arrays: score, anc_comp (decibans),
+    rho, lpr, elpr ([R, 2], coding first), status, n_evals."""
+    st = {"fixed": PCSF_STRATEGY_FIXED, "mle": PCSF_STRATEGY_MLE}[strategy]
+    R = batch.n_regions
+    out = dict(score=np.zeros(R), anc_comp=np.zeros(R), rho=np.zeros((R, 2)),
+               lpr=np.zeros((R, 2)), elpr=np.zeros((R, 2)),
+               status=np.zeros(R, np.int32), n_evals=np.zeros(R, np.int32))
+    batch.model.ctx.check(lib().pcsf_score(batch.h, st, precision, _dp(out["score"]),
+                                           _dp(out["anc_comp"]), _dp(out["rho"]),
+                                           _dp(out["lpr"]), _dp(out["elpr"]),
+                                           _i32(out["status"]), _i32(out["n_evals"])))
+    return out
 
 
 def score_fixed(batch: Batch, precision=PCSF_FP32):

@@ -7,6 +7,7 @@
 #include <memory>
 #include <new>
 
+#include "host/maximize.hpp"
 #include "host/text.hpp"
 #include "internal.hpp"
 
@@ -50,10 +51,25 @@ struct ProgramBuilder {
   int max_depth = 0;
   std::vector<int> gemm_pos;  // index into ops (pairs) of each GEMM
 
+  // cherry node -> table index when messages above cherries come from tables
+  std::vector<int> table_of;
+
   ProgramBuilder(const std::vector<int>& l, const std::vector<int>& r, int nleaves)
       : lc(l), rc(r), nl(nleaves), need(l.size(), 0) {}
 
   bool leaf(int i) const { return i < nl; }
+  bool tabled(int i) const { return !table_of.empty() && table_of[i] >= 0; }
+
+  // walk(y, d) followed by the GEMM of branch y, or the table read replacing both
+  void message(int y, int d) {
+    if (tabled(y)) {
+      emit(OP_CHERRYT, d, lc[y], rc[y]);
+      ops[ops.size() - 2] = table_of[y];
+      return;
+    }
+    walk(y, d);
+    gemm(d, y);
+  }
 
   void compute_need() {
     for (int i = nl; i < static_cast<int>(lc.size()); ++i) {
@@ -71,11 +87,16 @@ struct ProgramBuilder {
   void emit(int op, int d, int a0, int a1) {
     ops.push_back(op | (d << 8));
     ops.push_back((a0 & 0xffff) | ((a1 & 0xffff) << 16));
+    ops.push_back(kNoBranch);
+    ops.push_back(0);
     max_depth = std::max(max_depth, d);
   }
 
+  int count() const { return static_cast<int>(ops.size() / kOpWords); }
+  int opcode(int k) const { return ops[kOpWords * k] & 0xff; }
+
   void gemm(int d, int br) {
-    gemm_pos.push_back(static_cast<int>(ops.size() / 2));
+    gemm_pos.push_back(count());
     emit(OP_GEMM, d, br, kNoBranch);
   }
 
@@ -85,15 +106,12 @@ struct ProgramBuilder {
       emit(OP_CHERRY, d, a, b);
     } else if (leaf(a) || leaf(b)) {
       const int x = leaf(a) ? a : b, y = leaf(a) ? b : a;
-      walk(y, d);
-      gemm(d, y);
+      message(y, d);
       emit(OP_LEAFMUL, d, x, 0);
     } else {
       const int first = (need[b] > need[a]) ? b : a, second = (first == a) ? b : a;
-      walk(first, d);
-      gemm(d, first);
-      walk(second, d + 1);
-      gemm(d + 1, second);
+      message(first, d);
+      message(second, d + 1);
       emit(OP_MULPOP, d, 0, 0);
     }
   }
@@ -105,24 +123,71 @@ struct ProgramBuilder {
     emit(OP_END, 0, 0, 0);
     // each GEMM prefetches the next GEMM's tile
     for (size_t k = 0; k + 1 < gemm_pos.size(); ++k) {
-      int32_t& w1 = ops[2 * gemm_pos[k] + 1];
+      int32_t& w1 = ops[kOpWords * gemm_pos[k] + 1];
       const int br = w1 & 0xffff;
-      const int nbr = ops[2 * gemm_pos[k + 1] + 1] & 0xffff;
+      const int nbr = ops[kOpWords * gemm_pos[k + 1] + 1] & 0xffff;
       w1 = br | (nbr << 16);
+    }
+    // the last GEMM before a gather op (with no gather op in between) issues
+    // that op's first leaf gather
+    int last_gemm = -1;
+    for (int k = 0; k < count(); ++k) {
+      const int op = opcode(k);
+      if (op == OP_GEMM) {
+        last_gemm = k;
+      } else if (op == OP_CHERRYT) {
+        last_gemm = -1;
+      } else if (op == OP_CHERRY || op == OP_LEAFMUL) {
+        if (last_gemm >= 0) {
+          ops[kOpWords * last_gemm + 2] = ops[kOpWords * k + 1] & 0xffff;
+          ops[kOpWords * k] |= 1 << 16;
+        }
+        last_gemm = -1;
+      }
     }
   }
 };
 
 bool rho_valid(double r) { return r > 0.0 && std::isfinite(r); }
 
-int ensure_transitions(pcsf_model* m, const std::vector<double>& rho, float* ms_out) {
-  pcsf_ctx* c = m->ctx;
-  if (m->cached_rho == rho) {
-    if (ms_out) *ms_out = 0.f;
-    return PCSF_OK;
+hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0) {
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) {
+    if (e == hipSuccess) e = x;
+  };
+  const size_t n = (size_t)cap * nbr;
+  chk(dalloc(&ps.d_PT32, n * 4096));
+  chk(dalloc(&ps.d_Pfrag, n * kTileFloats));
+  chk(dalloc(&ps.d_lnPcc, n * 64));
+  chk(dalloc(&ps.d_rho, (size_t)cap));
+  chk(dalloc(&ps.d_active, (size_t)cap));
+  if (fp64) chk(dalloc(&ps.d_P64T, n * 4096));
+  if (nctab > 0) {
+    chk(dalloc(&ps.d_ctab, (size_t)cap * nctab * kTabCodes * kTabCodes * 64));
+    chk(dalloc(&ps.d_clnt, (size_t)cap * nctab * kTabCodes * kTabCodes));
   }
-  HIP_TRY(c, hipMemcpyAsync(m->d_rho, rho.data(), sizeof(double) * rho.size(),
-                            hipMemcpyHostToDevice, c->stream));
+  ps.cap = cap;
+  ps.fp64 = fp64;
+  return e;
+}
+
+void free_psets(PSets& ps) {
+  dfree(ps.d_P64T);
+  dfree(ps.d_PT32);
+  dfree(ps.d_Pfrag);
+  dfree(ps.d_lnPcc);
+  dfree(ps.d_rho);
+  dfree(ps.d_active);
+  dfree(ps.d_ctab);
+  dfree(ps.d_clnt);
+  ps.cap = 0;
+}
+
+// Runs the P builder over `nsets` sets whose rho/active are already on the
+// device; maps a recorded failure to the reference's message.
+int run_pbuild(pcsf_model* m, const PSets& ps, int nsets, const double* rho_host,
+               float* ms_out) {
+  pcsf_ctx* c = m->ctx;
   HIP_TRY(c, hipMemsetAsync(m->d_perr, 0, sizeof(int) * 8, c->stream));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing && ms_out) {
@@ -130,7 +195,7 @@ int ensure_transitions(pcsf_model* m, const std::vector<double>& rho, float* ms_
     HIP_TRY(c, hipEventCreate(&e1));
     HIP_TRY(c, hipEventRecord(e0, c->stream));
   }
-  HIP_TRY(c, launch_pbuild(m, m->d_rho, c->stream));
+  HIP_TRY(c, launch_pbuild(m, ps, nsets, c->stream));
   if (e1) HIP_TRY(c, hipEventRecord(e1, c->stream));
   int perr[8];
   double perrval = 0;
@@ -141,15 +206,14 @@ int ensure_transitions(pcsf_model* m, const std::vector<double>& rho, float* ms_
   if (e0) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    *ms_out = ms;
+    *ms_out += ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
   if (perr[0] != 0) {
-    m->cached_rho.clear();
     char buf[256];
-    const int br = perr[1], q = perr[2];
-    const double t = rho[q] * m->branch[br];
+    const int br = perr[1], set = perr[2];
+    const double t = rho_host[set] * m->branch[br];
     switch (perr[0]) {
       case 1:
         std::snprintf(buf, sizeof buf,
@@ -168,7 +232,135 @@ int ensure_transitions(pcsf_model* m, const std::vector<double>& rho, float* ms_
     }
     return set_err(c, PCSF_EFAIL, buf);
   }
+  return PCSF_OK;
+}
+
+int ensure_transitions(pcsf_model* m, const std::vector<double>& rho, float* ms_out) {
+  pcsf_ctx* c = m->ctx;
+  if (ms_out) *ms_out = 0.f;
+  if (m->cached_rho == rho) return PCSF_OK;
+  m->cached_rho.clear();
+  HIP_TRY(c, hipMemcpyAsync(m->fixed.d_rho, rho.data(), sizeof(double) * rho.size(),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemsetAsync(m->fixed.d_active, 1, (size_t)m->nm, c->stream));
+  int rc = run_pbuild(m, m->fixed, m->nm, rho.data(), ms_out);
+  if (rc) return rc;
+  HIP_TRY(c, launch_cherry_tables(m, m->fixed, m->nm, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
   m->cached_rho = rho;
+  return PCSF_OK;
+}
+
+// Builds the region-aligned copy of the batch's codes (first per-region call).
+int ensure_aligned(pcsf_batch* b) {
+  if (b->al.d_codes || b->nreg == 0) return PCSF_OK;
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  Aligned& al = b->al;
+  al.astart.assign(b->nreg, 0);
+  al.block0.assign(b->nreg, 0);
+  int64_t pos = 0;
+  for (int32_t r = 0; r < b->nreg; ++r) {
+    al.astart[r] = pos;
+    al.block0[r] = pos / kAlignCols;
+    const int64_t len = b->offsets[r + 1] - b->offsets[r];
+    pos += (len + kAlignCols - 1) / kAlignCols * kAlignCols;
+  }
+  al.ld = std::max<int64_t>(kGroupCols, (pos + kGroupCols - 1) / kGroupCols * kGroupCols);
+  std::vector<int32_t> br(al.ld / kAlignCols, -1);
+  for (int32_t r = 0; r < b->nreg; ++r) {
+    const int64_t len = b->offsets[r + 1] - b->offsets[r];
+    for (int64_t k = 0; k < (len + kAlignCols - 1) / kAlignCols; ++k) br[al.block0[r] + k] = r;
+  }
+  HIP_TRY(c, dalloc(&al.d_codes, (size_t)m->nl * al.ld));
+  HIP_TRY(c, dalloc(&al.d_block_region, br.size()));
+  HIP_TRY(c, dalloc(&al.d_astart, al.astart.size()));
+  HIP_TRY(c, dalloc(&b->d_alogz, (size_t)m->nm * al.ld));
+  HIP_TRY(c, dalloc(&b->d_acolelpr, (size_t)m->nm * al.ld));
+  HIP_TRY(c, hipMemcpy(al.d_block_region, br.data(), br.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(al.d_astart, al.astart.data(), al.astart.size() * 8,
+                       hipMemcpyHostToDevice));
+  HIP_TRY(c, launch_align_codes(b, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PCSF_OK;
+}
+
+// Bytes of one transition set on the device.
+size_t set_bytes(const pcsf_model* m, bool fp64) {
+  return (size_t)m->nbr * (4096 * 4 + kTileFloats * 4 + 64 * 8 + (fp64 ? 4096 * 8 : 0));
+}
+
+// Forward with one tree scale per (region, model): rho[r*nm+q].  active
+// (optional, r*nm+q) restricts the work to the listed pairs; results of the
+// others are left as they were.  Regions go through in chunks whose
+// transition sets fit the memory budget.
+int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int precision) {
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  const int nm = m->nm;
+  int rc = ensure_aligned(b);
+  if (rc) return rc;
+  const bool fp64 = precision == PCSF_FP64;
+  const size_t budget = (size_t)8 << 30;
+  int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, fp64) / nm * nm);
+  cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
+  if (b->rs.cap < cap || (fp64 && !b->rs.fp64)) {
+    free_psets(b->rs);
+    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, fp64));
+  }
+  const int chunk_regions = b->rs.cap / nm;
+  int variant = m->mfma_variant;
+  if (16 * kMfmaVariants[variant].wpg > kAlignCols) variant = 1;
+  std::vector<uint8_t> act;
+  b->t_pbuild = b->t_prune = b->t_reduce = 0.f;
+  for (int32_t r0 = 0; r0 < b->nreg; r0 += chunk_regions) {
+    const int32_t nr = std::min<int32_t>(chunk_regions, b->nreg - r0);
+    act.assign((size_t)nr * nm, 1);
+    bool any = false;
+    for (size_t k = 0; k < act.size(); ++k) {
+      if (active) act[k] = active[(size_t)r0 * nm + k] ? 1 : 0;
+      any = any || act[k];
+    }
+    if (!any) continue;
+    RegionChunk ch;
+    ch.r0 = r0;
+    ch.nreg = nr;
+    ch.block0 = b->al.block0[r0];
+    const int64_t end_block =
+        (r0 + nr < b->nreg) ? b->al.block0[r0 + nr] : (b->al.ld / kAlignCols);
+    ch.nblocks = end_block - ch.block0;
+    ch.sets = &b->rs;
+    HIP_TRY(c, hipMemcpyAsync(b->rs.d_rho, rho + (size_t)r0 * nm, sizeof(double) * nr * nm,
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(b->rs.d_active, act.data(), act.size(), hipMemcpyHostToDevice,
+                              c->stream));
+    float tp = 0.f;
+    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, &tp);
+    if (rc) {
+      char buf[64];
+      std::snprintf(buf, sizeof buf, " (region block from %d)", r0);
+      c->err += buf;
+      return rc;
+    }
+    b->t_pbuild += tp;
+    if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
+    hipError_t e = fp64 ? launch_prune_fp64_regions(m, b, ch, c->stream)
+                        : launch_prune_mfma_regions(m, b, variant, ch, c->stream);
+    if (e == hipErrorInvalidValue && fp64)
+      return set_err(c, PCSF_EINVAL, "FP64 kernel: tree workspace exceeds 64 KiB of LDS");
+    HIP_TRY(c, e);
+    if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[1], c->stream));
+    HIP_TRY(c, launch_region_reduce_aligned(b, ch, c->stream));
+    if (c->timing) {
+      HIP_TRY(c, hipEventRecord(b->ev[2], c->stream));
+      HIP_TRY(c, hipEventSynchronize(b->ev[2]));
+      float a = 0.f, r = 0.f;
+      HIP_TRY(c, hipEventElapsedTime(&a, b->ev[0], b->ev[1]));
+      HIP_TRY(c, hipEventElapsedTime(&r, b->ev[1], b->ev[2]));
+      b->t_prune += a;
+      b->t_reduce += r;
+    }
+  }
   return PCSF_OK;
 }
 
@@ -215,6 +407,36 @@ void* pcsf_ctx_stream(pcsf_ctx* c) { return c ? reinterpret_cast<void*>(c->strea
 int pcsf_ctx_synchronize(pcsf_ctx* c) {
   if (!c) return PCSF_EINVAL;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PCSF_OK;
+}
+
+int pcsf_program(int n, const int32_t* children, int with_tables, int32_t* ops, int cap,
+                 int* n_ops, int* slots) {
+  if (n < 3 || n % 2 == 0 || !children || !n_ops) return PCSF_EINVAL;
+  const int nl = (n + 1) / 2;
+  std::vector<int> lc(n, -1), rc(n, -1), par(n, -1);
+  for (int i = nl; i < n; ++i) {
+    const int a = children[(i - nl) * 2], b = children[(i - nl) * 2 + 1];
+    if (a < 0 || b < 0 || a >= i || b >= i || a == b || par[a] >= 0 || par[b] >= 0)
+      return PCSF_EINVAL;
+    lc[i] = a;
+    rc[i] = b;
+    par[a] = par[b] = i;
+  }
+  ProgramBuilder pb(lc, rc, nl);
+  if (with_tables) {
+    pb.table_of.assign(n, -1);
+    int t = 0;
+    for (int i = nl; i < n - 1; ++i)
+      if (lc[i] < nl && rc[i] < nl) pb.table_of[i] = t++;
+  }
+  pb.build(n - 1);
+  *n_ops = pb.count();
+  if (slots) *slots = pb.max_depth + 1;
+  if (ops) {
+    if (cap < (int)pb.ops.size()) return PCSF_EINVAL;
+    std::copy(pb.ops.begin(), pb.ops.end(), ops);
+  }
   return PCSF_OK;
 }
 
@@ -324,9 +546,24 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   ProgramBuilder pb(m->lc, m->rc, nl);
   pb.build(n - 1);
   m->prog = pb.ops;
-  m->slots = pb.max_depth + 1;
+  // cherries whose message feeds a GEMM (every cherry but a root cherry)
+  ProgramBuilder pt(m->lc, m->rc, nl);
+  pt.table_of.assign(n, -1);
+  for (int i = nl; i < n - 1; ++i)
+    if (m->lc[i] < nl && m->rc[i] < nl) {
+      pt.table_of[i] = (int)m->cherries.size() / 3;
+      m->cherries.push_back(i);
+      m->cherries.push_back(m->lc[i]);
+      m->cherries.push_back(m->rc[i]);
+    }
+  pt.build(n - 1);
+  m->prog_t = pt.ops;
+  m->first_gemm_br_t =
+      pt.gemm_pos.empty() ? kNoBranch : (m->prog_t[kOpWords * pt.gemm_pos[0] + 1] & 0xffff);
+  m->slots = std::max(pb.max_depth, pt.max_depth) + 1;
   m->n_gemm = static_cast<int>(pb.gemm_pos.size());
-  m->first_gemm_br = pb.gemm_pos.empty() ? kNoBranch : (m->prog[2 * pb.gemm_pos[0] + 1] & 0xffff);
+  m->first_gemm_br =
+      pb.gemm_pos.empty() ? kNoBranch : (m->prog[kOpWords * pb.gemm_pos[0] + 1] & 0xffff);
   if (m->slots > kMaxSlots)
     return set_err(c, PCSF_EINVAL, "tree needs more than 6 live partial tiles");
 
@@ -340,14 +577,11 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   HIP_TRY(c, dalloc(&m->d_lc, (size_t)n));
   HIP_TRY(c, dalloc(&m->d_rc, (size_t)n));
   HIP_TRY(c, dalloc(&m->d_prog, m->prog.size()));
-  HIP_TRY(c, dalloc(&m->d_rho, (size_t)nm));
+  HIP_TRY(c, dalloc(&m->d_prog_t, m->prog_t.size()));
+  HIP_TRY(c, dalloc(&m->d_cherries, std::max<size_t>(1, m->cherries.size())));
   HIP_TRY(c, dalloc(&m->d_perr, (size_t)8));
   HIP_TRY(c, dalloc(&m->d_perrval, (size_t)1));
-  const size_t pcount = (size_t)nm * m->nbr * 4096;
-  HIP_TRY(c, dalloc(&m->d_P64T, pcount));
-  HIP_TRY(c, dalloc(&m->d_PT32, pcount));
-  HIP_TRY(c, dalloc(&m->d_Pfrag, (size_t)nm * m->nbr * kTileFloats));
-  HIP_TRY(c, dalloc(&m->d_lnPcc, (size_t)nm * m->nbr * 64));
+  HIP_TRY(c, alloc_psets(m->fixed, nm, m->nbr, true, (int)m->cherries.size() / 3));
   std::vector<int32_t> lc32(m->lc.begin(), m->lc.end()), rc32(m->rc.begin(), m->rc.end());
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
@@ -361,6 +595,9 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   HIP_TRY(c, up(m->d_lc, lc32.data(), n * 4));
   HIP_TRY(c, up(m->d_rc, rc32.data(), n * 4));
   HIP_TRY(c, up(m->d_prog, m->prog.data(), m->prog.size() * 4));
+  HIP_TRY(c, up(m->d_prog_t, m->prog_t.data(), m->prog_t.size() * 4));
+  if (!m->cherries.empty())
+    HIP_TRY(c, up(m->d_cherries, m->cherries.data(), m->cherries.size() * 4));
   *out = guard.release();
   return PCSF_OK;
 }
@@ -376,13 +613,11 @@ void pcsf_model_destroy(pcsf_model* m) {
   dfree(m->d_lc);
   dfree(m->d_rc);
   dfree(m->d_prog);
-  dfree(m->d_rho);
+  dfree(m->d_prog_t);
+  dfree(m->d_cherries);
   dfree(m->d_perr);
   dfree(m->d_perrval);
-  dfree(m->d_P64T);
-  dfree(m->d_PT32);
-  dfree(m->d_Pfrag);
-  dfree(m->d_lnPcc);
+  free_psets(m->fixed);
   delete m;
 }
 
@@ -405,7 +640,7 @@ int pcsf_model_transition(pcsf_model* m, int model, double rho, double* P_out) {
   int rc = ensure_transitions(m, r, nullptr);
   if (rc) return rc;
   std::vector<double> T((size_t)m->nbr * 4096);
-  HIP_TRY(c, hipMemcpy(T.data(), m->d_P64T + (size_t)model * m->nbr * 4096, T.size() * 8,
+  HIP_TRY(c, hipMemcpy(T.data(), m->fixed.d_P64T + (size_t)model * m->nbr * 4096, T.size() * 8,
                        hipMemcpyDeviceToHost));
   for (int br = 0; br < m->nbr; ++br)
     for (int a = 0; a < 64; ++a)
@@ -432,6 +667,7 @@ int pcsf_batch_create(pcsf_model* m, const uint8_t* codes, int64_t ncols,
   b->m = m;
   b->ncols = ncols;
   b->nreg = nreg;
+  b->offsets.assign(off, off + nreg + 1);
   b->ld = std::max<int64_t>(kGroupCols, (ncols + kGroupCols - 1) / kGroupCols * kGroupCols);
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, dalloc(&b->d_codes, (size_t)m->nl * b->ld));
@@ -463,6 +699,12 @@ void pcsf_batch_destroy(pcsf_batch* b) {
   dfree(b->d_colelpr);
   dfree(b->d_loglik);
   dfree(b->d_elpr);
+  dfree(b->al.d_codes);
+  dfree(b->al.d_block_region);
+  dfree(b->al.d_astart);
+  dfree(b->d_alogz);
+  dfree(b->d_acolelpr);
+  free_psets(b->rs);
   for (auto& ev : b->ev)
     if (ev) (void)hipEventDestroy(ev);
   delete b;
@@ -476,8 +718,24 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
   (void)save_for_backward;
-  if (rho_per_region)
-    return set_err(c, PCSF_EINVAL, "pcsf_forward: per-region rho is not available yet");
+  if (precision != PCSF_FP64 && precision != PCSF_FP32)
+    return set_err(c, PCSF_EINVAL, "pcsf_forward: unknown precision");
+  if (rho_per_region) {
+    const size_t nr = (size_t)b->nreg * m->nm;
+    for (size_t k = 0; k < nr; ++k)
+      if (!rho_valid(rho[k]))
+        return set_err(c, PCSF_EINVAL,
+                       "CamlPaml.P14n.instantiate_tree: domain violation on variable 0");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = forward_regions(b, rho, nullptr, precision);
+    if (rc) return rc;
+    if (loglik && nr)
+      HIP_TRY(c, hipMemcpyAsync(loglik, b->d_loglik, nr * 8, hipMemcpyDeviceToHost, c->stream));
+    if (elpr && nr)
+      HIP_TRY(c, hipMemcpyAsync(elpr, b->d_elpr, nr * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return PCSF_OK;
+  }
   std::vector<double> r(rho, rho + m->nm);
   for (double x : r)
     if (!rho_valid(x))
@@ -560,6 +818,235 @@ int pcsf_set_variant(pcsf_model* m, int variant) {
   if (variant < 0 || variant >= kNumMfmaVariants)
     return set_err(m->ctx, PCSF_EINVAL, "pcsf_set_variant: no such launch variant");
   m->mfma_variant = variant;
+  return PCSF_OK;
+}
+
+namespace {
+
+struct PairState {
+  enum Phase { RETRY, BRENT, FINAL, DONE } phase = RETRY;
+  int retry = 0;         // retries done
+  double flo, fhi;       // lpr at lo, hi
+  double fx, ex, x;      // find_init's current point
+  pcsf::BrentMin br;
+  double pending = 0;    // abscissa to evaluate next
+  double res_x = 0, res_lpr = 0, res_elpr = 0;
+  int evals = 0;
+  int status = PCSF_OK;
+  // values already evaluated (x -> lpr, elpr), newest last
+  std::vector<double> cx, cl, ce;
+  void remember(double xx, double l, double e) {
+    cx.push_back(xx);
+    cl.push_back(l);
+    ce.push_back(e);
+  }
+  bool recall(double xx, double* l, double* e) const {
+    for (size_t k = cx.size(); k-- > 0;)
+      if (cx[k] == xx) {
+        *l = cl[k];
+        *e = ce[k];
+        return true;
+      }
+    return false;
+  }
+};
+
+// Evaluates lpr (and elpr) of every region and model at one shared tree scale.
+int eval_shared(pcsf_batch* b, double x, int precision, std::vector<double>& l,
+                std::vector<double>& e) {
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  std::vector<double> r(m->nm, x);
+  const size_t n = (size_t)b->nreg * m->nm;
+  l.assign(n, 0.0);
+  e.assign(n, 0.0);
+  int rc = pcsf_forward(b, r.data(), 0, precision, l.data(), e.data(), 0);
+  (void)c;
+  return rc;
+}
+
+// The reference's maximize_lpr for every (region, model) pair at once
+// (src/PhyloCSFModel.ml:84-99): Fit.find_init (init 1, [0.01, 10], log
+// space, at most 250 tries), then GSL Brent on -lpr until the bracket is
+// within 1% of the estimate, then lpr at the estimate.
+int score_mle(pcsf_batch* b, int precision, std::vector<PairState>& st) {
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  const int nm = m->nm;
+  const size_t np = (size_t)b->nreg * nm;
+  const double lo = 1e-2, hi = 10.0, init = 1.0, accuracy = 0.01;
+  std::vector<double> Llo, Elo, Lhi, Ehi, L1, E1, Lv, Ev;
+  int rc;
+  if ((rc = eval_shared(b, lo, precision, Llo, Elo))) return rc;
+  if ((rc = eval_shared(b, hi, precision, Lhi, Ehi))) return rc;
+  if ((rc = eval_shared(b, init, precision, L1, E1))) return rc;
+  pcsf::BrentMin probe;
+  probe.x_lo = lo;
+  probe.x_hi = hi;
+  const double v = probe.golden_point();
+  if ((rc = eval_shared(b, v, precision, Lv, Ev))) return rc;
+  const std::vector<double> retries = pcsf::find_init_retries(lo, hi, 250);
+  st.assign(np, PairState());
+  auto start_brent = [&](PairState& s, size_t k) {
+    // gsl_min_fminimizer_set: lower, upper, minimum must be finite and the
+    // endpoints must enclose the minimum of -lpr; then brent_init's f(v)
+    const double fmin = -s.fx, flo = -s.flo, fhi = -s.fhi, fv = -Lv[k];
+    if (!std::isfinite(fmin) || !std::isfinite(flo) || !std::isfinite(fhi) ||
+        !std::isfinite(fv) || !(fmin < flo && fmin < fhi)) {
+      s.status = PCSF_EFAIL;
+      s.phase = PairState::DONE;
+      return;
+    }
+    s.br.init(s.x, fmin, lo, flo, hi, fhi, fv);
+    s.br.propose();
+    s.pending = s.br.u;
+    s.phase = PairState::BRENT;
+  };
+  auto finish_at = [&](PairState& s, double x) {
+    double l, e;
+    if (s.recall(x, &l, &e)) {
+      s.res_x = x;
+      s.res_lpr = l;
+      s.res_elpr = e;
+      s.phase = PairState::DONE;
+    } else {
+      s.pending = x;
+      s.phase = PairState::FINAL;
+    }
+  };
+  for (size_t k = 0; k < np; ++k) {
+    PairState& s = st[k];
+    s.flo = Llo[k];
+    s.fhi = Lhi[k];
+    s.x = init;
+    s.fx = L1[k];
+    s.ex = E1[k];
+    s.evals = 4;
+    s.remember(lo, Llo[k], Elo[k]);
+    s.remember(hi, Lhi[k], Ehi[k]);
+    s.remember(v, Lv[k], Ev[k]);
+    s.remember(init, L1[k], E1[k]);
+    if (!(s.fx <= s.flo || s.fx <= s.fhi)) {
+      start_brent(s, k);
+    } else {
+      s.pending = retries[0];
+      s.phase = PairState::RETRY;
+    }
+  }
+  std::vector<double> rho(np, 1.0), L(np), E(np);
+  std::vector<uint8_t> act(np);
+  for (int round = 0; round < 100000; ++round) {
+    bool any = false;
+    for (size_t k = 0; k < np; ++k) {
+      act[k] = st[k].phase != PairState::DONE;
+      rho[k] = act[k] ? st[k].pending : 1.0;
+      any = any || act[k];
+    }
+    if (!any) break;
+    rc = forward_regions(b, rho.data(), act.data(), precision);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(E.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < np; ++k) {
+      if (!act[k]) continue;
+      PairState& s = st[k];
+      const double x = s.pending, l = L[k], e = E[k];
+      s.evals++;
+      s.remember(x, l, e);
+      switch (s.phase) {
+        case PairState::RETRY: {
+          s.x = x;
+          s.fx = l;
+          s.ex = e;
+          s.retry++;
+          // Fit.ml:36-47: after the 250th try the boundary is taken whatever it gave
+          if (s.retry == 250) {
+            finish_at(s, (s.flo > s.fhi) ? lo : hi);
+          } else if (!(s.fx <= s.flo || s.fx <= s.fhi)) {
+            start_brent(s, k);
+          } else {
+            s.pending = retries[s.retry];
+          }
+          break;
+        }
+        case PairState::BRENT: {
+          const double fu = -l;
+          if (!std::isfinite(fu)) {
+            s.status = PCSF_EFAIL;
+            s.phase = PairState::DONE;
+            break;
+          }
+          s.br.accept(fu);
+          if ((s.br.x_hi - s.br.x_lo) / s.br.x_min > accuracy) {
+            s.br.propose();
+            s.pending = s.br.u;
+          } else {
+            finish_at(s, s.br.x_min);
+          }
+          break;
+        }
+        case PairState::FINAL:
+          s.res_x = x;
+          s.res_lpr = l;
+          s.res_elpr = e;
+          s.phase = PairState::DONE;
+          break;
+        default:
+          break;
+      }
+    }
+  }
+  return PCSF_OK;
+}
+
+}  // namespace
+
+int pcsf_score(pcsf_batch* b, int strategy, int precision, double* score, double* anc_comp,
+               double* rho_out, double* lpr_out, double* elpr_out, int32_t* status,
+               int32_t* n_evals) {
+  if (!b) return PCSF_EINVAL;
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  if (m->nm != 2)
+    return set_err(c, PCSF_EINVAL, "pcsf_score: needs exactly two models (coding, noncoding)");
+  const int R = b->nreg;
+  const double db = 10.0 / std::log(10.0);
+  std::vector<double> l(2 * (size_t)R), e(2 * (size_t)R), r(2 * (size_t)R, 1.0);
+  std::vector<int32_t> stt(R, PCSF_OK), ev(R, 0);
+  if (strategy == PCSF_STRATEGY_FIXED) {
+    const double one[2] = {1.0, 1.0};
+    int rc = pcsf_forward(b, one, 0, precision, l.data(), e.data(), 0);
+    if (rc) return rc;
+    for (int i = 0; i < R; ++i) ev[i] = 2;
+  } else if (strategy == PCSF_STRATEGY_MLE) {
+    std::vector<PairState> st;
+    int rc = score_mle(b, precision, st);
+    if (rc) return rc;
+    for (int i = 0; i < R; ++i)
+      for (int q = 0; q < 2; ++q) {
+        const PairState& s = st[(size_t)i * 2 + q];
+        l[i * 2 + q] = s.res_lpr;
+        e[i * 2 + q] = s.res_elpr;
+        r[i * 2 + q] = s.res_x;
+        ev[i] += s.evals;
+        if (s.status) stt[i] = s.status;
+      }
+  } else {
+    return set_err(c, PCSF_EINVAL, "pcsf_score: unknown strategy");
+  }
+  for (int i = 0; i < R; ++i) {
+    const bool ok = stt[i] == PCSF_OK;
+    // score = db (lpr_c - lpr_n), anc_comp = db (elpr_c - elpr_n)
+    // (src/PhyloCSFModel.ml:113,122-136)
+    if (score) score[i] = ok ? db * (l[2 * i] - l[2 * i + 1]) : NAN;
+    if (anc_comp) anc_comp[i] = ok ? db * (e[2 * i] - e[2 * i + 1]) : NAN;
+    if (status) status[i] = stt[i];
+    if (n_evals) n_evals[i] = ev[i];
+  }
+  if (rho_out) std::copy(r.begin(), r.end(), rho_out);
+  if (lpr_out) std::copy(l.begin(), l.end(), lpr_out);
+  if (elpr_out) std::copy(e.begin(), e.end(), elpr_out);
   return PCSF_OK;
 }
 

@@ -11,20 +11,46 @@
 
 // Op program of the fused whole-tree kernel: a post-order walk of the tree
 // that keeps at most `slots` partial-likelihood tiles live in registers.
-// word0 = opcode | depth << 8, word1 = arg0 | arg1 << 16.
+// Four words per op: word0 = opcode | depth << 8 | use_prefetch << 16,
+// word1 = arg0 | arg1 << 16, word2 = leaf whose gather a GEMM issues ahead
+// (kNoBranch if none), word3 = 0.  use_prefetch marks a gather op whose
+// first leaf (arg0) was issued by the preceding GEMM.
 enum PcsfOp : int {
-  OP_CHERRY = 1,   // slot[d] = gather(leaf a) * gather(leaf b); rescale
-  OP_GEMM = 2,     // slot[d] = P_a slot[d]  (MFMA, P tile staged in LDS); arg1 = next GEMM branch
-  OP_LEAFMUL = 3,  // slot[d] *= gather(leaf a); rescale
+  OP_CHERRY = 1,   // slot[d] = gather(leaf a0) * gather(leaf a1); rescale
+  OP_GEMM = 2,     // slot[d] = P_a0 slot[d] (MFMA, P tile in LDS); a1 = next GEMM's branch
+  OP_LEAFMUL = 3,  // slot[d] *= gather(leaf a0); rescale
   OP_MULPOP = 4,   // slot[d] *= slot[d+1]; rescale
   OP_ROOT = 5,     // write ln z and elpr for slot[d]
   OP_END = 6,
+  OP_CHERRYT = 7,  // slot[d] = message of cherry node w2 above it, from its table at the
+                   // codes of leaves a0, a1 (replaces CHERRY + GEMM; fixed scales only)
 };
+constexpr int kTabCodes = 65;  // table index per leaf: codon 0..63, 64 = `Marginalize
 constexpr int kNoBranch = 0xFFFF;
+constexpr int kOpWords = 4;
 constexpr int kMaxSlots = 6;
 constexpr int kTileCols = 16;   // columns per wave
 constexpr int kGroupCols = 128; // columns per workgroup (8 waves)
 constexpr int kTileFloats = 4096 + 64;  // P tile staged in LDS: fragments + diagonal
+
+// Transition-matrix sets on the device: set s is model s % nm at tree scale
+// rho[s] (the fixed strategy uses nm sets; per-region scales use one set per
+// (region, model) of a chunk of regions).
+struct PSets {
+  int cap = 0;                 // sets allocated
+  bool fp64 = false;           // P64T allocated
+  double* d_P64T = nullptr;    // cap * nbr * 4096 FP64 P^T (P64T[b][a] = P[a][b])
+  float* d_PT32 = nullptr;     // cap * nbr * 4096 FP32 P^T[c][a] = P[a][c] / P[c][c]
+  float* d_Pfrag = nullptr;    // cap * nbr * kTileFloats: off-diagonal of P in MFMA
+                               //   A-fragment order, then the 64 values P[a][a] - 1
+  double* d_lnPcc = nullptr;   // cap * nbr * 64 FP64 ln P[c][c]
+  double* d_rho = nullptr;     // cap tree scales
+  uint8_t* d_active = nullptr; // cap flags: 0 = set not needed by this launch
+  // cherry tables (fixed scales only): message P_c (P_a[:,x] o P_b[:,y]) of
+  // cherry c above its two leaves a, b, normalised to max 1, and its ln max
+  float* d_ctab = nullptr;     // cap * nctab * 65 * 65 * 64
+  double* d_clnt = nullptr;    // cap * nctab * 65 * 65
+};
 
 struct pcsf_ctx {
   int device = 0;
@@ -41,8 +67,13 @@ struct pcsf_model {
   std::vector<double> branch;
   std::vector<pcsf::Eigen> eig;
   std::vector<double> pi;  // nm * 64
-  // program for the fused kernel
-  std::vector<int32_t> prog;  // pairs
+  // programs for the fused kernel: `prog` walks every node; `prog_t` reads
+  // the messages above cherries from tables (fixed scales)
+  std::vector<int32_t> prog, prog_t;
+  int32_t* d_prog_t = nullptr;
+  int first_gemm_br_t = kNoBranch;
+  std::vector<int32_t> cherries;  // cherry nodes with a GEMM above them (table order)
+  int32_t* d_cherries = nullptr;  // node, left leaf, right leaf per table
   int slots = 0;
   int n_gemm = 0;
   int mfma_variant = 1;  // launch shape of the fused kernel (pcsf_autotune)
@@ -57,16 +88,23 @@ struct pcsf_model {
   int32_t* d_lc = nullptr;     // n
   int32_t* d_rc = nullptr;
   int32_t* d_prog = nullptr;
-  // device: transition matrices at the cached tree scales
+  // device: transition matrices at the cached tree scales (set q = model q)
   std::vector<double> cached_rho;
-  double* d_P64T = nullptr;  // nm * nbr * 4096  FP64, P^T (col-major P)
-  float* d_PT32 = nullptr;   // nm * nbr * 4096  FP32, P^T[c][a] = P[a][c] / P[c][c]
-  float* d_Pfrag = nullptr;  // nm * nbr * 4160  FP32, off-diagonal of P in MFMA A-fragment
-                             //   order, then the 64 diagonal entries P[a][a] - 1
-  double* d_lnPcc = nullptr; // nm * nbr * 64    FP64, ln P[c][c]
-  double* d_rho = nullptr;   // nm
+  PSets fixed;
   int* d_perr = nullptr;     // error record of the P builder
   double* d_perrval = nullptr;
+};
+
+// Region-aligned copy of a batch's codes: every region starts on a multiple
+// of kAlignCols, so a workgroup's columns share one region (and one P set).
+constexpr int kAlignCols = 64;
+struct Aligned {
+  int64_t ld = 0;                   // aligned columns (multiple of kGroupCols)
+  uint8_t* d_codes = nullptr;       // nl * ld
+  int32_t* d_block_region = nullptr;// per kAlignCols block: region (-1 = none)
+  int64_t* d_astart = nullptr;      // per region: first aligned column
+  std::vector<int64_t> astart;      // host copy
+  std::vector<int64_t> block0;      // per region: first block
 };
 
 struct pcsf_batch {
@@ -81,20 +119,42 @@ struct pcsf_batch {
   double* d_elpr = nullptr;        // nreg * nm
   hipEvent_t ev[6] = {};
   float t_pbuild = 0, t_prune = 0, t_reduce = 0;
+  std::vector<int64_t> offsets;     // host copy
+  Aligned al;                       // built on the first per-region forward
+  PSets rs;                         // per-region transition sets (one chunk)
+  double* d_alogz = nullptr;        // nm * al.ld, per aligned column
+  double* d_acolelpr = nullptr;
+};
+
+// Per-region-scale launch description (one chunk of regions).
+struct RegionChunk {
+  int32_t r0 = 0, nreg = 0;         // regions [r0, r0 + nreg)
+  int64_t block0 = 0, nblocks = 0;  // aligned blocks covering them
+  const PSets* sets = nullptr;      // set (r - r0) * nm + q
 };
 
 // ---- kernel launchers (kernels/*.hip) --------------------------------------
 hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uint8_t* dst,
                                   int64_t ld, hipStream_t s);
-hipError_t launch_pbuild(const pcsf_model* m, const double* d_rho, hipStream_t s);
+hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, hipStream_t s);
+hipError_t launch_cherry_tables(const pcsf_model* m, const PSets& sets, int nsets,
+                                hipStream_t s);
 hipError_t launch_prune_fp64(const pcsf_model* m, const pcsf_batch* b, hipStream_t s);
+hipError_t launch_prune_fp64_regions(const pcsf_model* m, const pcsf_batch* b,
+                                     const RegionChunk& ch, hipStream_t s);
+hipError_t launch_prune_mfma_regions(const pcsf_model* m, const pcsf_batch* b, int variant,
+                                     const RegionChunk& ch, hipStream_t s);
+hipError_t launch_align_codes(const pcsf_batch* b, hipStream_t s);
+hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& ch,
+                                        hipStream_t s);
 // Launch shapes of the fused kernel that the autotuner chooses among.
 struct MfmaVariant {
-  int wpg;   // waves per workgroup (16 columns each) sharing one LDS P tile
-  int minw;  // minimum waves per SIMD the compiler must allow (register cap)
+  int wpg;        // waves per workgroup (16 columns each) sharing one LDS P tile
+  int minw;       // minimum waves per SIMD the compiler must allow (register cap)
+  bool prefetch;  // GEMMs issue the next leaf gather ahead (+18 VGPRs)
   const char* what;
 };
-constexpr int kNumMfmaVariants = 5;
+constexpr int kNumMfmaVariants = 6;
 extern const MfmaVariant kMfmaVariants[kNumMfmaVariants];
 hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int variant,
                              hipStream_t s);

@@ -62,7 +62,76 @@ __global__ __launch_bounds__(256) void region_reduce_kernel(
   }
 }
 
+// Region-aligned copy: aligned column a of region r (astart[r] <= a <
+// astart[r] + len) holds packed column off[r] + (a - astart[r]); the rest of
+// the region's last block is `Marginalize.
+__global__ void align_codes_kernel(const uint8_t* __restrict__ src, int64_t ld, int nl,
+                                   const int32_t* __restrict__ block_region,
+                                   const int64_t* __restrict__ astart,
+                                   const int64_t* __restrict__ off, uint8_t* __restrict__ dst,
+                                   int64_t ald) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ald) return;
+  const int r = block_region[a / kAlignCols];
+  int64_t pc = -1;
+  if (r >= 0) {
+    const int64_t k = a - astart[r];
+    if (k < off[r + 1] - off[r]) pc = off[r] + k;
+  }
+  for (int l = 0; l < nl; ++l)
+    dst[(size_t)l * ald + a] = (pc >= 0) ? src[(size_t)l * ld + pc] : (uint8_t)PCSF_MARGINALIZE;
+}
+
+__global__ __launch_bounds__(256) void region_reduce_aligned_kernel(
+    const double* __restrict__ logz, const double* __restrict__ colelpr,
+    const int64_t* __restrict__ off, const int64_t* __restrict__ astart,
+    const uint8_t* __restrict__ active, int r0, int nm, int64_t ald,
+    double* __restrict__ loglik, double* __restrict__ elpr) {
+  __shared__ double s0[256], s1[256];
+  const int r = r0 + blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
+  if (active && !active[(size_t)blockIdx.x * nm + q]) return;
+  const int64_t lo = astart[r], hi = lo + (off[r + 1] - off[r]);
+  double a = 0.0, b = 0.0;
+  for (int64_t c = lo + tid; c < hi; c += 256) {
+    a += logz[(size_t)q * ald + c];
+    b += colelpr[(size_t)q * ald + c];
+  }
+  s0[tid] = a;
+  s1[tid] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      s0[tid] += s0[tid + w];
+      s1[tid] += s1[tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    loglik[(size_t)r * nm + q] = s0[0];
+    elpr[(size_t)r * nm + q] = s1[0];
+  }
+}
+
 }  // namespace
+
+hipError_t launch_align_codes(const pcsf_batch* b, hipStream_t s) {
+  const int64_t blocks = (b->al.ld + 255) / 256;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(align_codes_kernel, dim3((unsigned)blocks), dim3(256), 0, s, b->d_codes,
+                     b->ld, b->m->nl, b->al.d_block_region, b->al.d_astart, b->d_offsets,
+                     b->al.d_codes, b->al.ld);
+  return hipGetLastError();
+}
+
+hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& ch,
+                                        hipStream_t s) {
+  if (ch.nreg == 0) return hipSuccess;
+  dim3 grid(ch.nreg, b->m->nm);
+  hipLaunchKernelGGL(region_reduce_aligned_kernel, grid, dim3(256), 0, s, b->d_alogz,
+                     b->d_acolelpr, b->d_offsets, b->al.d_astart, ch.sets->d_active, ch.r0,
+                     b->m->nm, b->al.ld, b->d_loglik, b->d_elpr);
+  return hipGetLastError();
+}
 
 hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uint8_t* dst,
                                   int64_t ld, hipStream_t s) {

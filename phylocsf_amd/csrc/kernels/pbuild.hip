@@ -1,6 +1,7 @@
 // pbuild.hip -- FP64 transition matrices P(t) = V diag(exp(lam t)) V^-1 on the GPU.
 //
-// One workgroup per (branch, model).  Reproduces Q.Diag.real_to_Pt
+// One workgroup per (branch, set); set s is model s % nm at tree scale
+// rho[s] and is skipped when active[s] == 0.  Reproduces Q.Diag.real_to_Pt
 // (lib/CamlPaml/Q.ml:211-249): entries below zero are an error when
 // |x| > 1e-6 and are set to 0 otherwise; the row sum must be within 1e-6 of
 // 1; the diagonal is replaced by 1 - sum(off-diagonal).  The first failure
@@ -24,13 +25,14 @@ constexpr double kTol = 1e-6;
 __global__ __launch_bounds__(256) void pbuild_kernel(
     const double* __restrict__ V, const double* __restrict__ Vinv,
     const double* __restrict__ lam, const double* __restrict__ branch,
-    const double* __restrict__ rho, int nbr, double* __restrict__ P64T,
-    float* __restrict__ PT32, float* __restrict__ Pfrag, double* __restrict__ lnPcc,
-    int* __restrict__ perr, double* __restrict__ perrval) {
+    const double* __restrict__ rho, const uint8_t* __restrict__ active, int nm, int nbr,
+    double* __restrict__ P64T, float* __restrict__ PT32, float* __restrict__ Pfrag,
+    double* __restrict__ lnPcc, int* __restrict__ perr, double* __restrict__ perrval) {
   __shared__ double W[64 * 64];  // diag(exp(lam t)) V^-1, then P
   __shared__ double Vs[64 * 64];
-  const int br = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
-  const double t = rho[q] * branch[br];
+  const int br = blockIdx.x, set = blockIdx.y, q = set % nm, tid = threadIdx.x;
+  if (active && !active[set]) return;
+  const double t = rho[set] * branch[br];
   const double* Vq = V + (size_t)q * 4096;
   const double* Viq = Vinv + (size_t)q * 4096;
   for (int e = tid; e < 4096; e += 256) {
@@ -61,7 +63,7 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
   __syncthreads();
   if (t < 0.0) {
     if (tid == 0 && atomicCAS(perr, 0, 4) == 0) {
-      perr[1] = br; perr[2] = q; perr[3] = 0; perr[4] = 0; *perrval = t;
+      perr[1] = br; perr[2] = set; perr[3] = 0; perr[4] = 0; *perrval = t;
     }
   }
   if (tid < 64) {  // row fix-up, sequential over j like the reference
@@ -82,20 +84,20 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
     if (code == 0 && fabs(tot - 1.0) > kTol) { code = 2; bad = tot; }
     if (code == 0 && !(diag <= 1.0 && diag > 0.0)) { code = 3; bad = diag; }
     if (code != 0 && atomicCAS(perr, 0, code) == 0) {
-      perr[1] = br; perr[2] = q; perr[3] = i; perr[4] = jbad; *perrval = bad;
+      perr[1] = br; perr[2] = set; perr[3] = i; perr[4] = jbad; *perrval = bad;
     }
     Vs[i * 64 + i] = diag;
   }
   __syncthreads();
-  const size_t base = ((size_t)q * nbr + br) * 4096;
+  const size_t base = ((size_t)set * nbr + br) * 4096;
   for (int e = tid; e < 4096; e += 256) {
     const int b = e >> 6, a = e & 63;  // transposed: [b][a] = P[a][b]
     const double p = Vs[a * 64 + b];
-    P64T[base + e] = p;
+    if (P64T) P64T[base + e] = p;
     PT32[base + e] = (float)(p / Vs[b * 64 + b]);
   }
-  if (tid < 64) lnPcc[((size_t)q * nbr + br) * 64 + tid] = log(Vs[tid * 64 + tid]);
-  const size_t fbase = ((size_t)q * nbr + br) * kTileFloats;
+  if (tid < 64) lnPcc[((size_t)set * nbr + br) * 64 + tid] = log(Vs[tid * 64 + tid]);
+  const size_t fbase = ((size_t)set * nbr + br) * kTileFloats;
   // fragment order of v_mfma_f32_16x16x4f32's A operand with the K index
   // permuted to match the accumulator layout (prune_mfma.hip):
   //   e = ((mo*4 + mb)*64 + lane)*4 + v
@@ -109,12 +111,58 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
   if (tid < 64) Pfrag[fbase + 4096 + tid] = (float)(Vs[tid * 64 + tid] - 1.0);
 }
 
+// Cherry tables: for cherry c with leaves a, b (branches into them from c),
+// T[x][y] = P_c (P_a[:, x] o P_b[:, y]) for codons x, y or `Marginalize (64,
+// the all-ones leaf vector), in FP64, stored normalised to max 1 (FP32) with
+// its ln max (FP64).  One 64-thread workgroup per (x, y) and (set, table).
+__global__ __launch_bounds__(64) void cherry_table_kernel(const double* __restrict__ P64T,
+                                                         const int32_t* __restrict__ cher,
+                                                         int nctab, int nbr,
+                                                         float* __restrict__ ctab,
+                                                         double* __restrict__ clnt) {
+  __shared__ double al[64];
+  __shared__ double mx[64];
+  const int xy = blockIdx.x, x = xy / kTabCodes, y = xy % kTabCodes;
+  const int st = blockIdx.y, set = st / nctab, t = st % nctab, k = threadIdx.x;
+  const int c = cher[3 * t], a = cher[3 * t + 1], b = cher[3 * t + 2];
+  const double* Pa = P64T + ((size_t)set * nbr + a) * 4096;  // Pa[col * 64 + row]
+  const double* Pb = P64T + ((size_t)set * nbr + b) * 4096;
+  const double* Pc = P64T + ((size_t)set * nbr + c) * 4096;
+  al[k] = (x == 64 ? 1.0 : Pa[x * 64 + k]) * (y == 64 ? 1.0 : Pb[y * 64 + k]);
+  __syncthreads();
+  double m = 0.0;
+  for (int j = 0; j < 64; ++j) m = fma(Pc[j * 64 + k], al[j], m);  // row k of P_c al
+  mx[k] = m;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (k < w) mx[k] = fmax(mx[k], mx[k + w]);
+    __syncthreads();
+  }
+  const double top = mx[0];
+  const size_t base = ((size_t)st * kTabCodes * kTabCodes + xy);
+  const bool ones = (x == 64 && y == 64);  // P 1 = 1 exactly
+  ctab[base * 64 + k] = ones ? 1.0f : (float)(m / top);
+  if (k == 0) clnt[base] = ones ? 0.0 : log(top);
+}
+
 }  // namespace
 
-hipError_t launch_pbuild(const pcsf_model* m, const double* d_rho, hipStream_t s) {
-  dim3 grid(m->nbr, m->nm);
+hipError_t launch_cherry_tables(const pcsf_model* m, const PSets& sets, int nsets,
+                                hipStream_t s) {
+  const int nct = (int)m->cherries.size() / 3;
+  if (nct == 0 || nsets == 0) return hipSuccess;
+  dim3 grid(kTabCodes * kTabCodes, nsets * nct);
+  hipLaunchKernelGGL(cherry_table_kernel, grid, dim3(64), 0, s, sets.d_P64T, m->d_cherries, nct,
+                     m->nbr, sets.d_ctab, sets.d_clnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, hipStream_t s) {
+  if (nsets <= 0) return hipSuccess;
+  dim3 grid(m->nbr, nsets);
   hipLaunchKernelGGL(pbuild_kernel, grid, dim3(256), 0, s, m->d_V, m->d_Vinv, m->d_lam,
-                     m->d_branch, d_rho, m->nbr, m->d_P64T, m->d_PT32, m->d_Pfrag, m->d_lnPcc,
-                     m->d_perr, m->d_perrval);
+                     m->d_branch, sets.d_rho, sets.d_active, m->nm, m->nbr,
+                     sets.fp64 ? sets.d_P64T : nullptr, sets.d_PT32, sets.d_Pfrag,
+                     sets.d_lnPcc, m->d_perr, m->d_perrval);
   return hipGetLastError();
 }
