@@ -125,11 +125,16 @@ int64_t pcsf_batch_columns(const pcsf_batch *b);
 int pcsf_forward(pcsf_batch *b, const double *rho, int rho_per_region, int precision,
                  double *loglik, double *elpr, int save_for_backward);
 
-/* Backward of the last forward: given dL/dloglik (n_regions*n_models),
- * returns dL/drho (n_regions*n_models) and optionally dL/dt_br summed over
- * the batch (n_models*(n_nodes-1)).  The gradient the reference's PhyloEM
- * defines (lib/CamlPaml/PhyloEM.ml:140-182, d ln z/dt through branch
- * posteriors, PhyloLik.ml:140-174). */
+/* Backward of the last forward on this batch (same tree scales): given
+ * dL/dloglik (n_regions*n_models), returns dL/drho (n_regions*n_models) and,
+ * if grad_branch is non-NULL, dL/dt_br per region and model
+ * (n_regions*n_models*(n_nodes-1); t_br = rho * b_br is the branch length,
+ * so dL/drho = sum_br b_br dL/dt_br).  The gradient the reference's PhyloEM
+ * defines (lib/CamlPaml/PhyloEM.ml:140-182: d ELL/dt through the branch
+ * posteriors of PhyloLik.ml:140-174, equal to d ln z/dt), computed by the
+ * outside (preorder) pass of PhyloLik.ensure_beta (PhyloLik.ml:96-120) in
+ * FP64; the inside pass is recomputed, so forward saves no partials.
+ * Per-branch sums use FP64 atomics (not bitwise reproducible run to run). */
 int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
                   double *grad_branch);
 
@@ -145,6 +150,10 @@ int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
  * n_evals = likelihood evaluations per region.  Any output may be NULL. */
 #define PCSF_STRATEGY_FIXED 0
 #define PCSF_STRATEGY_MLE 1
+/* mle with the adjoint: root of d lpr / d ln rho on [0.01, 10] by Illinois
+ * regula falsi (to 1e-7 in ln rho), boundary when the gradient keeps its sign;
+ * finds the maximum the reference's 1%-tolerance Brent approximates. */
+#define PCSF_STRATEGY_MLE_GRAD 2
 int pcsf_score(pcsf_batch *b, int strategy, int precision, double *score, double *anc_comp,
                double *rho, double *lpr, double *elpr, int32_t *status, int32_t *n_evals);
 

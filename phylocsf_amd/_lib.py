@@ -31,6 +31,7 @@ EXPORTED = [
 ]
 PCSF_STRATEGY_FIXED = 0
 PCSF_STRATEGY_MLE = 1
+PCSF_STRATEGY_MLE_GRAD = 2
 
 _lib = None
 

@@ -19,7 +19,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from ._lib import (PCSF_FP32, PCSF_FP64, PCSF_MARGINALIZE, PCSF_STRATEGY_FIXED,
-                   PCSF_STRATEGY_MLE, InvalidArgument, lib, raise_for)
+                   PCSF_STRATEGY_MLE, PCSF_STRATEGY_MLE_GRAD, InvalidArgument, lib, raise_for)
 
 DB = 10.0 / np.log(10.0)  # PhyloCSFModel.db (src/PhyloCSFModel.ml:113)
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -263,6 +263,18 @@ class Batch:
                                                 precision, _dp(ll), _dp(el), 0))
         return ll, el
 
+    def backward(self, grad_loglik, branches=False):
+        """Gradients of the last forward: -> (dL/drho [R, nm], dL/dt_br [R, nm, n-1] or None)
+        for dL/dloglik = grad_loglik [R, nm] (outside pass, FP64)."""
+        nm = self.model.nm
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(grad_loglik, dtype=np.float64),
+                                                 (self.n_regions, nm)))
+        grho = np.zeros((self.n_regions, nm))
+        gbr = np.zeros((self.n_regions, nm, self.model.tree.n - 1)) if branches else None
+        self.model.ctx.check(lib().pcsf_backward(self.h, _dp(g), _dp(grho),
+                                                 _dp(gbr) if branches else None))
+        return grho, gbr
+
     def autotune(self, reps=3):
         """Time every launch shape of the fused kernel on this batch and keep the
         fastest for this model: -> (chosen index, [ms per shape])."""
@@ -292,10 +304,12 @@ class Batch:
 def score(batch: Batch, strategy="fixed", precision=PCSF_FP32):
     """PhyloCSFModel.score for every region of a batch (src/PhyloCSFModel.ml:138-146).
 
-    strategy "fixed" (llr_FixedLik 1.0) or "mle" (llr_MaxLik: Fit.find_init + GSL
-    Brent per region and model).  -> dict of arrays: score, anc_comp (decibans),
+    strategy "fixed" (llr_FixedLik 1.0), "mle" (llr_MaxLik: Fit.find_init + GSL
+    Brent per region and model) or "mle_grad" (root of d lpr / d ln rho through the
+    adjoint kernel).  -> dict of arrays: score, anc_comp (decibans),
     rho, lpr, elpr ([R, 2], coding first), status, n_evals."""
-    st = {"fixed": PCSF_STRATEGY_FIXED, "mle": PCSF_STRATEGY_MLE}[strategy]
+    st = {"fixed": PCSF_STRATEGY_FIXED, "mle": PCSF_STRATEGY_MLE,
+          "mle_grad": PCSF_STRATEGY_MLE_GRAD}[strategy]
     R = batch.n_regions
     out = dict(score=np.zeros(R), anc_comp=np.zeros(R), rho=np.zeros((R, 2)),
                lpr=np.zeros((R, 2)), elpr=np.zeros((R, 2)),

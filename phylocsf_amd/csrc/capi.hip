@@ -364,6 +364,60 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
   return PCSF_OK;
 }
 
+// Adjoint pass at tree scales rho (nreg * nm) over the pairs in `active`
+// (optional): per-region lpr into d_loglik and d lpr / d rho into d_elpr
+// (the backward reuses the elpr buffers), and, if d_gbr is given, the
+// per-region sums of d ln z / d t_br (nreg * nm * nbr, accumulated).
+int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, double* d_gbr) {
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  const int nm = m->nm;
+  int rc = ensure_aligned(b);
+  if (rc) return rc;
+  const size_t budget = (size_t)8 << 30;
+  int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, true) / nm * nm);
+  cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
+  if (b->rs.cap < cap || !b->rs.fp64) {
+    free_psets(b->rs);
+    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, true));
+  }
+  const int chunk_regions = b->rs.cap / nm;
+  std::vector<uint8_t> act;
+  for (int32_t r0 = 0; r0 < b->nreg; r0 += chunk_regions) {
+    const int32_t nr = std::min<int32_t>(chunk_regions, b->nreg - r0);
+    act.assign((size_t)nr * nm, 1);
+    bool any = false;
+    for (size_t k = 0; k < act.size(); ++k) {
+      if (active) act[k] = active[(size_t)r0 * nm + k] ? 1 : 0;
+      any = any || act[k];
+    }
+    if (!any) continue;
+    RegionChunk ch;
+    ch.r0 = r0;
+    ch.nreg = nr;
+    ch.block0 = b->al.block0[r0];
+    const int64_t end_block =
+        (r0 + nr < b->nreg) ? b->al.block0[r0 + nr] : (b->al.ld / kAlignCols);
+    ch.nblocks = end_block - ch.block0;
+    ch.sets = &b->rs;
+    HIP_TRY(c, hipMemcpyAsync(b->rs.d_rho, rho + (size_t)r0 * nm, sizeof(double) * nr * nm,
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(b->rs.d_active, act.data(), act.size(), hipMemcpyHostToDevice,
+                              c->stream));
+    float tp = 0.f;
+    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, &tp);
+    if (rc) return rc;
+    hipError_t e = launch_adjoint_fp64(
+        m, b, ch, d_gbr ? d_gbr + (size_t)r0 * nm * m->nbr : nullptr, c->stream);
+    if (e == hipErrorInvalidValue)
+      return set_err(c, PCSF_EINVAL, "adjoint kernel: tree workspace exceeds 64 KiB of LDS");
+    HIP_TRY(c, e);
+    HIP_TRY(c, launch_region_reduce_aligned(b, ch, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PCSF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -530,7 +584,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
       return set_err(c, PCSF_EINVAL, "CamlPaml.PhyloModel.make: negative or missing branch");
   m->eig.resize(nm);
   m->pi.assign(pi, pi + 64 * nm);
-  std::vector<double> V(nm * 4096), Vi(nm * 4096), lam(nm * 64), lnpi(nm * 64);
+  std::vector<double> V(nm * 4096), Vi(nm * 4096), lam(nm * 64), lnpi(nm * 64), QT(nm * 4096);
   for (int q = 0; q < nm; ++q) {
     std::vector<double> Q(4096);
     std::string msg;
@@ -538,6 +592,8 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
       return set_err(c, PCSF_EFAIL, msg);
     if (!pcsf::eigen_reversible(64, Q.data(), pi + q * 64, &m->eig[q], &msg))
       return set_err(c, PCSF_EINVAL, msg);
+    for (int a = 0; a < 64; ++a)
+      for (int k = 0; k < 64; ++k) QT[q * 4096 + k * 64 + a] = Q[a * 64 + k];
     std::copy(m->eig[q].V.begin(), m->eig[q].V.end(), V.begin() + q * 4096);
     std::copy(m->eig[q].Vinv.begin(), m->eig[q].Vinv.end(), Vi.begin() + q * 4096);
     std::copy(m->eig[q].lam.begin(), m->eig[q].lam.end(), lam.begin() + q * 64);
@@ -573,6 +629,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   HIP_TRY(c, dalloc(&m->d_lam, lam.size()));
   HIP_TRY(c, dalloc(&m->d_pi, m->pi.size()));
   HIP_TRY(c, dalloc(&m->d_lnpi, lnpi.size()));
+  HIP_TRY(c, dalloc(&m->d_QT, QT.size()));
   HIP_TRY(c, dalloc(&m->d_branch, m->branch.size()));
   HIP_TRY(c, dalloc(&m->d_lc, (size_t)n));
   HIP_TRY(c, dalloc(&m->d_rc, (size_t)n));
@@ -591,6 +648,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   HIP_TRY(c, up(m->d_lam, lam.data(), lam.size() * 8));
   HIP_TRY(c, up(m->d_pi, m->pi.data(), m->pi.size() * 8));
   HIP_TRY(c, up(m->d_lnpi, lnpi.data(), lnpi.size() * 8));
+  HIP_TRY(c, up(m->d_QT, QT.data(), QT.size() * 8));
   HIP_TRY(c, up(m->d_branch, m->branch.data(), m->branch.size() * 8));
   HIP_TRY(c, up(m->d_lc, lc32.data(), n * 4));
   HIP_TRY(c, up(m->d_rc, rc32.data(), n * 4));
@@ -609,6 +667,7 @@ void pcsf_model_destroy(pcsf_model* m) {
   dfree(m->d_lam);
   dfree(m->d_pi);
   dfree(m->d_lnpi);
+  dfree(m->d_QT);
   dfree(m->d_branch);
   dfree(m->d_lc);
   dfree(m->d_rc);
@@ -729,6 +788,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
     HIP_TRY(c, hipSetDevice(c->device));
     int rc = forward_regions(b, rho, nullptr, precision);
     if (rc) return rc;
+    b->last_rho.assign(rho, rho + nr);
     if (loglik && nr)
       HIP_TRY(c, hipMemcpyAsync(loglik, b->d_loglik, nr * 8, hipMemcpyDeviceToHost, c->stream));
     if (elpr && nr)
@@ -746,6 +806,9 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   int rc = ensure_transitions(m, r, &tp);
   if (rc) return rc;
   b->t_pbuild = tp;
+  b->last_rho.resize((size_t)b->nreg * m->nm);
+  for (int32_t i = 0; i < b->nreg; ++i)
+    for (int q = 0; q < m->nm; ++q) b->last_rho[(size_t)i * m->nm + q] = r[q];
   if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
   hipError_t e;
   if (precision == PCSF_FP64) {
@@ -1000,6 +1063,120 @@ int score_mle(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   return PCSF_OK;
 }
 
+// Gradient maximiser of lpr over the tree scale for every (region, model)
+// pair: a safeguarded root search on h(x) = d lpr / d ln rho over
+// [ln 0.01, ln 10] (the reference's bounds): bracket from rho = 1 toward the
+// side the gradient points to, the boundary when the gradient keeps its
+// sign there, then Illinois regula falsi to |dx| <= 1e-7.  Each step is one
+// adjoint launch giving lpr and d lpr / d rho; a final forward gives elpr.
+int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  const size_t np = (size_t)b->nreg * m->nm;
+  const double xlo = std::log(1e-2), xhi = std::log(10.0);
+  struct G {
+    int phase = 0;  // 0: at 1, 1: probing end, 2: bracketed, 3: done
+    double x = 0, h = 0;
+    double a = 0, ha = 0, bb = 0, hb = 0;
+    int side = 0;  // Illinois bookkeeping
+  };
+  std::vector<G> g(np);
+  st.assign(np, PairState());
+  std::vector<double> rho(np, 1.0), L(np), D(np);
+  std::vector<uint8_t> act(np, 1);
+  auto eval = [&]() -> int {
+    int rc = adjoint_regions(b, rho.data(), act.data(), nullptr);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(D.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost));
+    return PCSF_OK;
+  };
+  for (int it = 0; it < 200; ++it) {
+    bool any = false;
+    for (size_t k = 0; k < np; ++k) {
+      act[k] = g[k].phase != 3;
+      rho[k] = std::exp(g[k].x);
+      any = any || act[k];
+    }
+    if (!any) break;
+    int rc = eval();
+    if (rc) return rc;
+    for (size_t k = 0; k < np; ++k) {
+      if (!act[k]) continue;
+      G& s = g[k];
+      st[k].evals++;
+      const double h = std::exp(s.x) * D[k];  // d lpr / d ln rho
+      if (!std::isfinite(h) || !std::isfinite(L[k])) {
+        st[k].status = PCSF_EFAIL;
+        s.phase = 3;
+        continue;
+      }
+      if (s.phase == 0) {
+        s.a = s.x;
+        s.ha = h;
+        if (h == 0.0) {
+          s.phase = 3;
+        } else {
+          s.phase = 1;
+          s.x = (h > 0) ? xhi : xlo;
+        }
+      } else if (s.phase == 1) {
+        if ((s.ha > 0 && h >= 0) || (s.ha < 0 && h <= 0)) {
+          s.phase = 3;  // maximum on the boundary
+        } else {
+          s.bb = s.x;
+          s.hb = h;
+          s.phase = 2;
+          s.side = 0;
+        }
+      } else if (s.phase == 2) {
+        if (h == 0.0) {
+          s.phase = 3;
+        } else if ((h > 0) == (s.ha > 0)) {
+          s.a = s.x;
+          s.ha = h;
+          if (s.side == -1) s.hb *= 0.5;
+          s.side = -1;
+        } else {
+          s.bb = s.x;
+          s.hb = h;
+          if (s.side == 1) s.ha *= 0.5;
+          s.side = 1;
+        }
+      }
+      if (s.phase == 2) {
+        if (std::fabs(s.bb - s.a) <= 1e-7) {
+          s.x = 0.5 * (s.a + s.bb);
+          s.phase = 3;
+        } else {
+          double xn = (s.a * s.hb - s.bb * s.ha) / (s.hb - s.ha);
+          const double lo = std::min(s.a, s.bb), hi = std::max(s.a, s.bb);
+          if (!(xn > lo && xn < hi)) xn = 0.5 * (s.a + s.bb);
+          if (std::fabs(xn - s.x) <= 1e-9) {
+            s.phase = 3;  // converged on the same point
+          } else {
+            s.x = xn;
+          }
+        }
+      }
+    }
+  }
+  // lpr and elpr at the estimates, in the requested precision
+  for (size_t k = 0; k < np; ++k) rho[k] = std::exp(g[k].x);
+  std::vector<uint8_t> all(np, 1);
+  int rc = forward_regions(b, rho.data(), all.data(), precision);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(D.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < np; ++k) {
+    st[k].res_x = rho[k];
+    st[k].res_lpr = L[k];
+    st[k].res_elpr = D[k];
+    st[k].evals++;
+  }
+  return PCSF_OK;
+}
+
 }  // namespace
 
 int pcsf_score(pcsf_batch* b, int strategy, int precision, double* score, double* anc_comp,
@@ -1019,9 +1196,10 @@ int pcsf_score(pcsf_batch* b, int strategy, int precision, double* score, double
     int rc = pcsf_forward(b, one, 0, precision, l.data(), e.data(), 0);
     if (rc) return rc;
     for (int i = 0; i < R; ++i) ev[i] = 2;
-  } else if (strategy == PCSF_STRATEGY_MLE) {
+  } else if (strategy == PCSF_STRATEGY_MLE || strategy == PCSF_STRATEGY_MLE_GRAD) {
     std::vector<PairState> st;
-    int rc = score_mle(b, precision, st);
+    int rc = (strategy == PCSF_STRATEGY_MLE) ? score_mle(b, precision, st)
+                                             : score_mle_grad(b, precision, st);
     if (rc) return rc;
     for (int i = 0; i < R; ++i)
       for (int q = 0; q < 2; ++q) {
@@ -1050,9 +1228,40 @@ int pcsf_score(pcsf_batch* b, int strategy, int precision, double* score, double
   return PCSF_OK;
 }
 
-int pcsf_backward(pcsf_batch* b, const double*, double*, double*) {
-  if (!b) return PCSF_EINVAL;
-  return set_err(b->m->ctx, PCSF_EINVAL, "pcsf_backward: adjoint kernel not available yet");
+int pcsf_backward(pcsf_batch* b, const double* grad_loglik, double* grad_rho,
+                  double* grad_branch) {
+  if (!b || !grad_loglik) return PCSF_EINVAL;
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  const size_t np = (size_t)b->nreg * m->nm;
+  if (b->last_rho.size() != np)
+    return set_err(c, PCSF_EINVAL, "pcsf_backward: no forward on this batch to differentiate");
+  HIP_TRY(c, hipSetDevice(c->device));
+  double* d_gbr = nullptr;
+  const size_t ngb = np * m->nbr;
+  if (grad_branch && ngb) {
+    HIP_TRY(c, dalloc(&d_gbr, ngb));
+    HIP_TRY(c, hipMemsetAsync(d_gbr, 0, ngb * 8, c->stream));
+  }
+  int rc = adjoint_regions(b, b->last_rho.data(), nullptr, d_gbr);
+  std::vector<double> dl(np), gb;
+  if (!rc && np) {
+    hipError_t e = hipMemcpy(dl.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && d_gbr) {
+      gb.resize(ngb);
+      e = hipMemcpy(gb.data(), d_gbr, ngb * 8, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) rc = set_err(c, PCSF_EDEVICE, hipGetErrorString(e));
+  }
+  dfree(d_gbr);
+  if (rc) return rc;
+  for (size_t k = 0; k < np; ++k) {
+    if (grad_rho) grad_rho[k] = grad_loglik[k] * dl[k];
+    if (grad_branch)
+      for (int br = 0; br < m->nbr; ++br)
+        grad_branch[k * m->nbr + br] = grad_loglik[k] * gb[k * m->nbr + br];
+  }
+  return PCSF_OK;
 }
 
 int pcsf_batch_device_results(pcsf_batch* b, const double** d_loglik, const double** d_elpr) {

@@ -84,6 +84,7 @@ struct pcsf_model {
   double* d_lam = nullptr;   // nm * 64
   double* d_pi = nullptr;    // nm * 64
   double* d_lnpi = nullptr;  // nm * 64, ln pi
+  double* d_QT = nullptr;    // nm * 4096, Q transposed (QT[k][a] = Q[a][k])
   double* d_branch = nullptr;  // nbr
   int32_t* d_lc = nullptr;     // n
   int32_t* d_rc = nullptr;
@@ -121,6 +122,7 @@ struct pcsf_batch {
   float t_pbuild = 0, t_prune = 0, t_reduce = 0;
   std::vector<int64_t> offsets;     // host copy
   Aligned al;                       // built on the first per-region forward
+  std::vector<double> last_rho;     // tree scales of the last forward (nreg * nm)
   PSets rs;                         // per-region transition sets (one chunk)
   double* d_alogz = nullptr;        // nm * al.ld, per aligned column
   double* d_acolelpr = nullptr;
@@ -145,6 +147,8 @@ hipError_t launch_prune_fp64_regions(const pcsf_model* m, const pcsf_batch* b,
 hipError_t launch_prune_mfma_regions(const pcsf_model* m, const pcsf_batch* b, int variant,
                                      const RegionChunk& ch, hipStream_t s);
 hipError_t launch_align_codes(const pcsf_batch* b, hipStream_t s);
+hipError_t launch_adjoint_fp64(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
+                               double* grad_branch, hipStream_t s);
 hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& ch,
                                         hipStream_t s);
 // Launch shapes of the fused kernel that the autotuner chooses among.
