@@ -32,6 +32,29 @@ sys.path.insert(0, ROOT)
 METRIC = "codon-columns/s, batched MFMA vs plain FP64 GPU, at |delta score|<=1e-3 dB"
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0
+PEAK_F16_TFLOPS = 2500.0   # MI355X dense FP16/BF16 matrix
+# name -> (C-ABI precision, dtype, peak of the MFMA it runs on, MFMA ops per algorithmic flop)
+PRECISIONS = {"fp32": (1, "fp32", PEAK_FP32_TFLOPS, 1.0),
+              "fp16s": (2, "fp16x2 split (hi+lo), fp32 accumulate", PEAK_F16_TFLOPS, 3.0),
+              "fp16": (3, "fp16, fp32 accumulate", PEAK_F16_TFLOPS, 1.0),
+              "bf16": (4, "bf16, fp32 accumulate", PEAK_F16_TFLOPS, 1.0)}
+
+
+def profiled_traffic(precision, shape):
+    """HBM bytes per launch of the fused kernel at this precision and launch
+    shape, as measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (scaled by a plain copy kernel) committed under profiles/; None when that
+    configuration has not been profiled."""
+    path = os.path.join(ROOT, "profiles", "prune_mfma_58mammals_c2.md")
+    key = "traffic %s shape%d " % (precision, shape)
+    try:
+        for line in open(path):
+            line = line.strip()
+            if line.startswith(key):
+                return float(line[len(key):])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -112,6 +135,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="threads of the CPU baseline (the GPU host's share is 16 cores)")
     ap.add_argument("--regions", type=int, default=None, help="override region count")
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fp32",
+                    help="GEMM precision of the batched MFMA path")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (regions shard "
@@ -136,24 +161,25 @@ def main():
 
     # ---- batched FP32 MFMA path (the measured value); the autotuner picks the
     # launch shape before the timed region
-    batch.forward(1.0, P.PCSF_FP32)
-    chosen, shape_ms = batch.autotune(reps=2)
+    prec, dtype, peak, mfma_per_flop = PRECISIONS[args.precision]
+    batch.forward(1.0, prec)
+    chosen, shape_ms = batch.autotune(reps=2, precision=prec)
     log("autotune: " + ", ".join("v%d %.2f ms" % (i, t) for i, t in enumerate(shape_ms)) +
         " -> v%d" % chosen)
     for _ in range(args.warmup):
-        batch.forward(1.0, P.PCSF_FP32)
+        batch.forward(1.0, prec)
     ctx.synchronize()
     kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ll32, el32 = batch.forward(1.0, P.PCSF_FP32)
+        ll32, el32 = batch.forward(1.0, prec)
         kernel_ms.append(batch.timing()["prune_ms"])
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     value = ncols * args.steps / elapsed
     ms_per_step = 1000 * elapsed / args.steps
-    log("FP32 MFMA: %.3f ms/step, %.3e col/s, kernel %.3f ms" % (ms_per_step, value,
-                                                                np.mean(kernel_ms)))
+    log("%s MFMA: %.3f ms/step, %.3e col/s, kernel %.3f ms" % (args.precision, ms_per_step, value,
+                                                              np.mean(kernel_ms)))
 
     # ---- plain FP64 GPU kernel on the same batch (the 50x denominator)
     fp64_rate, dmax = None, None
@@ -175,8 +201,19 @@ def main():
     flops_per_col = 16384.0 * (nl - 2)          # 2 models x (L-2) x 2*64^2
     kavg_s = float(np.mean(kernel_ms)) / 1000.0
     achieved = flops_per_col * ncols / kavg_s / 1e12
-    roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP32_TFLOPS, traffic=None,
+    # executed matrix-core work: GEMMs left after the cherry tables, times the
+    # MFMA products per GEMM (three for split FP16)
+    from phylocsf_amd.api import program
+    ops, _ = program(model.tree, with_tables=1)
+    n_gemm = int(np.sum((np.asarray(ops).reshape(-1, 4)[:, 0] & 0xff) == 2))
+    exec_per_col = 2 * n_gemm * 2 * 64 * 64 * mfma_per_flop
+    executed = exec_per_col * ncols / kavg_s / 1e12
+    roofline = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s",
+                    frac=achieved / peak, traffic=(profiled_traffic(args.precision, chosen)
+                                          if args.regions is None else None),
+                    algorithmic_bytes=float(ncols * (nl + 32)),
+                    executed_mfma_tflops=executed, executed_frac=executed / peak,
+                    gemms_per_column_per_model=n_gemm,
                     kernel="prune_mfma_kernel", kernel_ms=float(np.mean(kernel_ms)),
                     launch_shape=int(chosen),
                     autotune_ms=[round(float(t), 3) for t in shape_ms],
@@ -199,14 +236,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {
             "workload": "58mammals, %d regions, lengths U{50..2000}, 50/50 coding/noncoding, "
                         "gap p=0.1, 1%% N, fixed strategy (seed 2)" % wl.n_regions,
             "codon_columns": ncols,
             "models": 2,
-            "precision": "FP32 MFMA, FP64 P(t) and log-scaling",
+            "precision": "%s MFMA GEMM inputs, FP32 partials, FP64 P(t) and log-scaling"
+                         % args.precision,
         },
         "plain_fp64_gpu": {"value": fp64_rate, "unit": "codon-columns/s",
                            "speedup_of_value": value / fp64_rate if fp64_rate else None},
