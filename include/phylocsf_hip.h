@@ -39,9 +39,17 @@ extern "C" {
 
 #define PCSF_MARGINALIZE 255
 
-/* arithmetic of the pruning contraction */
-#define PCSF_FP64 0 /* plain FP64 kernel: one column per workgroup (GEMV) */
-#define PCSF_FP32 1 /* fused whole-tree FP32 MFMA kernel, FP64 log-scaling */
+/* arithmetic of the pruning contraction.  Every MFMA precision keeps live
+ * partials in FP32 registers with per-column power-of-two rescaling, the
+ * diagonal of P in FP32 VALU arithmetic, and FP64 log sums; they differ in
+ * how P and the partial enter the matrix cores. */
+#define PCSF_FP64 0  /* plain FP64 kernel: one column per workgroup (GEMV) */
+#define PCSF_FP32 1  /* fused whole-tree kernel, FP32 MFMA (16x16x4) */
+#define PCSF_FP16S 2 /* split FP16: P and partial as hi + lo FP16 pairs (scaled
+                        by 2^14), hi*hi + hi*lo + lo*hi on FP16 MFMA (16x16x32),
+                        FP32 accumulation */
+#define PCSF_FP16 3  /* FP16 P and partial (scaled by 2^14), FP32 accumulation */
+#define PCSF_BF16 4  /* BF16 P and partial, FP32 accumulation */
 
 typedef struct pcsf_ctx pcsf_ctx;
 typedef struct pcsf_model pcsf_model;
@@ -157,15 +165,26 @@ int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
 int pcsf_score(pcsf_batch *b, int strategy, int precision, double *score, double *anc_comp,
                double *rho, double *lpr, double *elpr, int32_t *status, int32_t *n_evals);
 
-/* Autotuner of the fused FP32 kernel: times every launch shape (waves per
- * workgroup sharing one LDS P tile x register cap) on this batch, `reps`
- * launches each, keeps the fastest on the batch's model for later forwards
- * and returns the per-shape milliseconds in ms_out[PCSF_N_VARIANTS].
- * No counterpart in the reference (a tuning step of this implementation). */
+/* Autotuner of the fused kernel at one MFMA precision (PCSF_FP32 ..
+ * PCSF_BF16): times every launch shape (waves per workgroup sharing one LDS
+ * P tile x register cap x gather prefetch) on this batch at tree scale 1,
+ * `reps` launches each, keeps the fastest on the batch's model for later
+ * forwards at that precision and returns the per-shape milliseconds in
+ * ms_out[PCSF_N_VARIANTS].  No counterpart in the reference (a tuning step
+ * of this implementation). */
 #define PCSF_N_VARIANTS 6
-int pcsf_autotune(pcsf_batch *b, int reps, double *ms_out, int *chosen);
-/* Select a launch shape by index (0..PCSF_N_VARIANTS-1) without timing. */
-int pcsf_set_variant(pcsf_model *m, int variant);
+int pcsf_autotune(pcsf_batch *b, int precision, int reps, double *ms_out, int *chosen);
+/* Select a launch shape by index (0..PCSF_N_VARIANTS-1) for one MFMA
+ * precision without timing. */
+int pcsf_set_variant(pcsf_model *m, int precision, int variant);
+/* Tree-specialised kernels: the library carries straight-line op programs
+ * generated at build time for the shipped trees; a model whose tree matches
+ * one uses it (FP32 and split-FP16 GEMMs, launch shapes 1, 2 and 4) instead
+ * of the op-program interpreter.  enable = 1 / 0 allows / forbids them
+ * (-1 leaves the setting); *available (may be NULL) gets 1 if the plain
+ * program is specialised, | 2 if the cherry-table program is.  No
+ * counterpart in the reference. */
+int pcsf_model_specialized(pcsf_model *m, int enable, int *available);
 
 /* Device pointers of the last forward's per-region results (FP64,
  * n_regions*n_models each) and the kernel times when timing is enabled. */

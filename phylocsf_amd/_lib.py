@@ -19,6 +19,9 @@ PCSF_ENOMEM = -4
 PCSF_MARGINALIZE = 255
 PCSF_FP64 = 0
 PCSF_FP32 = 1
+PCSF_FP16S = 2
+PCSF_FP16 = 3
+PCSF_BF16 = 4
 
 EXPORTED = [
     "pcsf_ctx_create", "pcsf_ctx_destroy", "pcsf_last_error", "pcsf_ctx_set_timing",
@@ -27,6 +30,7 @@ EXPORTED = [
     "pcsf_batch_create", "pcsf_batch_destroy", "pcsf_batch_columns", "pcsf_forward",
     "pcsf_backward", "pcsf_batch_device_results", "pcsf_batch_last_timing",
     "pcsf_lik_columns", "pcsf_simulate", "pcsf_autotune", "pcsf_set_variant", "pcsf_score",
+    "pcsf_model_specialized",
     "pcsf_program",
 ]
 PCSF_STRATEGY_FIXED = 0
@@ -105,8 +109,9 @@ def lib():
     L.pcsf_lik_columns.argtypes = [vp, C.c_int, C.c_int, i32p, dp, dp, i32p, C.c_int64, dp, dp]
     L.pcsf_simulate.argtypes = [vp, C.c_int32, i64p, i32p, dp, C.c_int, C.c_double,
                                 C.c_double, C.c_double, C.c_uint64, C.c_int, u8p]
-    L.pcsf_autotune.argtypes = [vp, C.c_int, dp, ip]
-    L.pcsf_set_variant.argtypes = [vp, C.c_int]
+    L.pcsf_autotune.argtypes = [vp, C.c_int, C.c_int, dp, ip]
+    L.pcsf_set_variant.argtypes = [vp, C.c_int, C.c_int]
+    L.pcsf_model_specialized.argtypes = [vp, C.c_int, ip]
     L.pcsf_score.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, dp, dp, i32p, i32p]
     L.pcsf_program.argtypes = [C.c_int, i32p, C.c_int, i32p, C.c_int, ip, ip]
     _lib = L

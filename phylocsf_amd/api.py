@@ -18,8 +18,9 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from ._lib import (PCSF_FP32, PCSF_FP64, PCSF_MARGINALIZE, PCSF_STRATEGY_FIXED,
-                   PCSF_STRATEGY_MLE, PCSF_STRATEGY_MLE_GRAD, InvalidArgument, lib, raise_for)
+from ._lib import (PCSF_BF16, PCSF_FP16, PCSF_FP16S, PCSF_FP32, PCSF_FP64,  # noqa: F401
+                   PCSF_MARGINALIZE, PCSF_STRATEGY_FIXED, PCSF_STRATEGY_MLE,
+                   PCSF_STRATEGY_MLE_GRAD, InvalidArgument, lib, raise_for)
 
 DB = 10.0 / np.log(10.0)  # PhyloCSFModel.db (src/PhyloCSFModel.ml:113)
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -180,8 +181,17 @@ class PhyloCSFModel:
         return cls(ctx, tree, [import_parameters(prefix + "_coding.ECM"),
                                import_parameters(prefix + "_noncoding.ECM")])
 
-    def set_variant(self, v):
-        self.ctx.check(lib().pcsf_set_variant(self.h, int(v)))
+    def specialized(self, enable=None):
+        """Allow (True) or forbid (False) the build-time tree-specialised
+        kernels for this model; None leaves the setting.  Returns which of the
+        model's programs have one: {"plain": bool, "tables": bool}."""
+        av = C.c_int(0)
+        self.ctx.check(lib().pcsf_model_specialized(self.h, -1 if enable is None else int(bool(enable)),
+                                                    C.byref(av)))
+        return {"plain": bool(av.value & 1), "tables": bool(av.value & 2)}
+
+    def set_variant(self, v, precision=PCSF_FP32):
+        self.ctx.check(lib().pcsf_set_variant(self.h, int(precision), int(v)))
 
     def info(self):
         n, nl, nm, sl = C.c_int(), C.c_int(), C.c_int(), C.c_int()
@@ -275,12 +285,14 @@ class Batch:
                                                  _dp(gbr) if branches else None))
         return grho, gbr
 
-    def autotune(self, reps=3):
-        """Time every launch shape of the fused kernel on this batch and keep the
-        fastest for this model: -> (chosen index, [ms per shape])."""
+    def autotune(self, reps=3, precision=PCSF_FP32):
+        """Time every launch shape of the fused kernel at one MFMA precision on
+        this batch and keep the fastest for this model and precision:
+        -> (chosen index, [ms per shape])."""
         ms = np.zeros(6)
         ch = C.c_int(0)
-        self.model.ctx.check(lib().pcsf_autotune(self.h, reps, _dp(ms), C.byref(ch)))
+        self.model.ctx.check(lib().pcsf_autotune(self.h, int(precision), reps, _dp(ms),
+                                                 C.byref(ch)))
         return ch.value, ms
 
     def timing(self):

@@ -8,6 +8,7 @@
 #include <new>
 
 #include "host/maximize.hpp"
+#include "host/program.hpp"
 #include "host/text.hpp"
 #include "internal.hpp"
 
@@ -38,115 +39,6 @@ void dfree(T*& p) {
   if (p) (void)hipFree(p);
   p = nullptr;
 }
-
-// Compile the tree into the fused kernel's op program: post-order, the
-// subtree needing more live tiles first (Sethi-Ullman order), so the number
-// of live 64x32 tiles is minimal.
-struct ProgramBuilder {
-  const std::vector<int>& lc;
-  const std::vector<int>& rc;
-  int nl;
-  std::vector<int> need;
-  std::vector<int32_t> ops;
-  int max_depth = 0;
-  std::vector<int> gemm_pos;  // index into ops (pairs) of each GEMM
-
-  // cherry node -> table index when messages above cherries come from tables
-  std::vector<int> table_of;
-
-  ProgramBuilder(const std::vector<int>& l, const std::vector<int>& r, int nleaves)
-      : lc(l), rc(r), nl(nleaves), need(l.size(), 0) {}
-
-  bool leaf(int i) const { return i < nl; }
-  bool tabled(int i) const { return !table_of.empty() && table_of[i] >= 0; }
-
-  // walk(y, d) followed by the GEMM of branch y, or the table read replacing both
-  void message(int y, int d) {
-    if (tabled(y)) {
-      emit(OP_CHERRYT, d, lc[y], rc[y]);
-      ops[ops.size() - 2] = table_of[y];
-      return;
-    }
-    walk(y, d);
-    gemm(d, y);
-  }
-
-  void compute_need() {
-    for (int i = nl; i < static_cast<int>(lc.size()); ++i) {
-      const int a = lc[i], b = rc[i];
-      if (leaf(a) && leaf(b)) need[i] = 1;
-      else if (leaf(a)) need[i] = std::max(1, need[b]);
-      else if (leaf(b)) need[i] = std::max(1, need[a]);
-      else {
-        const int x = need[a], y = need[b];
-        need[i] = std::max(std::max(x, y), std::min(x, y) + 1);
-      }
-    }
-  }
-
-  void emit(int op, int d, int a0, int a1) {
-    ops.push_back(op | (d << 8));
-    ops.push_back((a0 & 0xffff) | ((a1 & 0xffff) << 16));
-    ops.push_back(kNoBranch);
-    ops.push_back(0);
-    max_depth = std::max(max_depth, d);
-  }
-
-  int count() const { return static_cast<int>(ops.size() / kOpWords); }
-  int opcode(int k) const { return ops[kOpWords * k] & 0xff; }
-
-  void gemm(int d, int br) {
-    gemm_pos.push_back(count());
-    emit(OP_GEMM, d, br, kNoBranch);
-  }
-
-  void walk(int i, int d) {
-    const int a = lc[i], b = rc[i];
-    if (leaf(a) && leaf(b)) {
-      emit(OP_CHERRY, d, a, b);
-    } else if (leaf(a) || leaf(b)) {
-      const int x = leaf(a) ? a : b, y = leaf(a) ? b : a;
-      message(y, d);
-      emit(OP_LEAFMUL, d, x, 0);
-    } else {
-      const int first = (need[b] > need[a]) ? b : a, second = (first == a) ? b : a;
-      message(first, d);
-      message(second, d + 1);
-      emit(OP_MULPOP, d, 0, 0);
-    }
-  }
-
-  void build(int root) {
-    compute_need();
-    walk(root, 0);
-    emit(OP_ROOT, 0, 0, 0);
-    emit(OP_END, 0, 0, 0);
-    // each GEMM prefetches the next GEMM's tile
-    for (size_t k = 0; k + 1 < gemm_pos.size(); ++k) {
-      int32_t& w1 = ops[kOpWords * gemm_pos[k] + 1];
-      const int br = w1 & 0xffff;
-      const int nbr = ops[kOpWords * gemm_pos[k + 1] + 1] & 0xffff;
-      w1 = br | (nbr << 16);
-    }
-    // the last GEMM before a gather op (with no gather op in between) issues
-    // that op's first leaf gather
-    int last_gemm = -1;
-    for (int k = 0; k < count(); ++k) {
-      const int op = opcode(k);
-      if (op == OP_GEMM) {
-        last_gemm = k;
-      } else if (op == OP_CHERRYT) {
-        last_gemm = -1;
-      } else if (op == OP_CHERRY || op == OP_LEAFMUL) {
-        if (last_gemm >= 0) {
-          ops[kOpWords * last_gemm + 2] = ops[kOpWords * k + 1] & 0xffff;
-          ops[kOpWords * k] |= 1 << 16;
-        }
-        last_gemm = -1;
-      }
-    }
-  }
-};
 
 bool rho_valid(double r) { return r > 0.0 && std::isfinite(r); }
 
@@ -185,7 +77,7 @@ void free_psets(PSets& ps) {
 
 // Runs the P builder over `nsets` sets whose rho/active are already on the
 // device; maps a recorded failure to the reference's message.
-int run_pbuild(pcsf_model* m, const PSets& ps, int nsets, const double* rho_host,
+int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int fmt,
                float* ms_out) {
   pcsf_ctx* c = m->ctx;
   HIP_TRY(c, hipMemsetAsync(m->d_perr, 0, sizeof(int) * 8, c->stream));
@@ -195,7 +87,8 @@ int run_pbuild(pcsf_model* m, const PSets& ps, int nsets, const double* rho_host
     HIP_TRY(c, hipEventCreate(&e1));
     HIP_TRY(c, hipEventRecord(e0, c->stream));
   }
-  HIP_TRY(c, launch_pbuild(m, ps, nsets, c->stream));
+  ps.fmt = -1;
+  HIP_TRY(c, launch_pbuild(m, ps, nsets, fmt, c->stream));
   if (e1) HIP_TRY(c, hipEventRecord(e1, c->stream));
   int perr[8];
   double perrval = 0;
@@ -232,20 +125,24 @@ int run_pbuild(pcsf_model* m, const PSets& ps, int nsets, const double* rho_host
     }
     return set_err(c, PCSF_EFAIL, buf);
   }
+  ps.fmt = fmt;
   return PCSF_OK;
 }
 
-int ensure_transitions(pcsf_model* m, const std::vector<double>& rho, float* ms_out) {
+// Transition sets and cherry tables of the fixed scales `rho` (one per
+// model) with P tiles in format fmt; rebuilt only when either changes.
+int ensure_transitions(pcsf_model* m, const std::vector<double>& rho, int fmt, float* ms_out) {
   pcsf_ctx* c = m->ctx;
   if (ms_out) *ms_out = 0.f;
-  if (m->cached_rho == rho) return PCSF_OK;
+  if (m->cached_rho == rho && m->fixed.fmt == fmt) return PCSF_OK;
+  const bool same_rho = m->cached_rho == rho;
   m->cached_rho.clear();
   HIP_TRY(c, hipMemcpyAsync(m->fixed.d_rho, rho.data(), sizeof(double) * rho.size(),
                             hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipMemsetAsync(m->fixed.d_active, 1, (size_t)m->nm, c->stream));
-  int rc = run_pbuild(m, m->fixed, m->nm, rho.data(), ms_out);
+  int rc = run_pbuild(m, m->fixed, m->nm, rho.data(), fmt, ms_out);
   if (rc) return rc;
-  HIP_TRY(c, launch_cherry_tables(m, m->fixed, m->nm, c->stream));
+  if (!same_rho) HIP_TRY(c, launch_cherry_tables(m, m->fixed, m->nm, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   m->cached_rho = rho;
   return PCSF_OK;
@@ -301,6 +198,7 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
   int rc = ensure_aligned(b);
   if (rc) return rc;
   const bool fp64 = precision == PCSF_FP64;
+  const int fmt = fp64 ? kFragF32 : frag_format(precision);
   const size_t budget = (size_t)8 << 30;
   int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, fp64) / nm * nm);
   cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
@@ -309,7 +207,7 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
     HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, fp64));
   }
   const int chunk_regions = b->rs.cap / nm;
-  int variant = m->mfma_variant;
+  int variant = fp64 ? 0 : m->mfma_variant[precision];
   if (16 * kMfmaVariants[variant].wpg > kAlignCols) variant = 1;
   std::vector<uint8_t> act;
   b->t_pbuild = b->t_prune = b->t_reduce = 0.f;
@@ -335,7 +233,7 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
     HIP_TRY(c, hipMemcpyAsync(b->rs.d_active, act.data(), act.size(), hipMemcpyHostToDevice,
                               c->stream));
     float tp = 0.f;
-    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, &tp);
+    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, fmt, &tp);
     if (rc) {
       char buf[64];
       std::snprintf(buf, sizeof buf, " (region block from %d)", r0);
@@ -345,7 +243,7 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
     b->t_pbuild += tp;
     if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
     hipError_t e = fp64 ? launch_prune_fp64_regions(m, b, ch, c->stream)
-                        : launch_prune_mfma_regions(m, b, variant, ch, c->stream);
+                        : launch_prune_mfma_regions(m, b, precision, variant, ch, c->stream);
     if (e == hipErrorInvalidValue && fp64)
       return set_err(c, PCSF_EINVAL, "FP64 kernel: tree workspace exceeds 64 KiB of LDS");
     HIP_TRY(c, e);
@@ -405,7 +303,7 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
     HIP_TRY(c, hipMemcpyAsync(b->rs.d_active, act.data(), act.size(), hipMemcpyHostToDevice,
                               c->stream));
     float tp = 0.f;
-    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, &tp);
+    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, kFragF32, &tp);
     if (rc) return rc;
     hipError_t e = launch_adjoint_fp64(
         m, b, ch, d_gbr ? d_gbr + (size_t)r0 * nm * m->nbr : nullptr, c->stream);
@@ -477,19 +375,12 @@ int pcsf_program(int n, const int32_t* children, int with_tables, int32_t* ops, 
     rc[i] = b;
     par[a] = par[b] = i;
   }
-  ProgramBuilder pb(lc, rc, nl);
-  if (with_tables) {
-    pb.table_of.assign(n, -1);
-    int t = 0;
-    for (int i = nl; i < n - 1; ++i)
-      if (lc[i] < nl && rc[i] < nl) pb.table_of[i] = t++;
-  }
-  pb.build(n - 1);
-  *n_ops = pb.count();
-  if (slots) *slots = pb.max_depth + 1;
+  pcsf::Program pg = pcsf::make_program(lc, rc, nl, with_tables != 0);
+  *n_ops = static_cast<int>(pg.ops.size() / kOpWords);
+  if (slots) *slots = pg.slots;
   if (ops) {
-    if (cap < (int)pb.ops.size()) return PCSF_EINVAL;
-    std::copy(pb.ops.begin(), pb.ops.end(), ops);
+    if (cap < (int)pg.ops.size()) return PCSF_EINVAL;
+    std::copy(pg.ops.begin(), pg.ops.end(), ops);
   }
   return PCSF_OK;
 }
@@ -599,27 +490,17 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
     std::copy(m->eig[q].lam.begin(), m->eig[q].lam.end(), lam.begin() + q * 64);
     for (int x = 0; x < 64; ++x) lnpi[q * 64 + x] = std::log(pi[q * 64 + x]);
   }
-  ProgramBuilder pb(m->lc, m->rc, nl);
-  pb.build(n - 1);
-  m->prog = pb.ops;
-  // cherries whose message feeds a GEMM (every cherry but a root cherry)
-  ProgramBuilder pt(m->lc, m->rc, nl);
-  pt.table_of.assign(n, -1);
-  for (int i = nl; i < n - 1; ++i)
-    if (m->lc[i] < nl && m->rc[i] < nl) {
-      pt.table_of[i] = (int)m->cherries.size() / 3;
-      m->cherries.push_back(i);
-      m->cherries.push_back(m->lc[i]);
-      m->cherries.push_back(m->rc[i]);
-    }
-  pt.build(n - 1);
+  const pcsf::Program pg = pcsf::make_program(m->lc, m->rc, nl, false);
+  const pcsf::Program pt = pcsf::make_program(m->lc, m->rc, nl, true);
+  m->prog = pg.ops;
   m->prog_t = pt.ops;
-  m->first_gemm_br_t =
-      pt.gemm_pos.empty() ? kNoBranch : (m->prog_t[kOpWords * pt.gemm_pos[0] + 1] & 0xffff);
-  m->slots = std::max(pb.max_depth, pt.max_depth) + 1;
-  m->n_gemm = static_cast<int>(pb.gemm_pos.size());
-  m->first_gemm_br =
-      pb.gemm_pos.empty() ? kNoBranch : (m->prog[kOpWords * pb.gemm_pos[0] + 1] & 0xffff);
+  m->cherries = pt.cherries;
+  m->first_gemm_br_t = pt.first_gemm_br;
+  m->first_gemm_br = pg.first_gemm_br;
+  m->slots = std::max(pg.slots, pt.slots);
+  m->n_gemm = pg.n_gemm;
+  m->tree_p = find_tree_program(m->prog, false);
+  m->tree_t = find_tree_program(m->prog_t, true);
   if (m->slots > kMaxSlots)
     return set_err(c, PCSF_EINVAL, "tree needs more than 6 live partial tiles");
 
@@ -696,7 +577,7 @@ int pcsf_model_transition(pcsf_model* m, int model, double rho, double* P_out) {
   if (!rho_valid(rho))
     return set_err(c, PCSF_EINVAL, "CamlPaml.P14n.instantiate_tree: domain violation on variable 0");
   std::vector<double> r(m->nm, rho);
-  int rc = ensure_transitions(m, r, nullptr);
+  int rc = ensure_transitions(m, r, m->fixed.fmt < 0 ? kFragF32 : m->fixed.fmt, nullptr);
   if (rc) return rc;
   std::vector<double> T((size_t)m->nbr * 4096);
   HIP_TRY(c, hipMemcpy(T.data(), m->fixed.d_P64T + (size_t)model * m->nbr * 4096, T.size() * 8,
@@ -777,7 +658,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
   (void)save_for_backward;
-  if (precision != PCSF_FP64 && precision != PCSF_FP32)
+  if (precision < PCSF_FP64 || precision > PCSF_BF16)
     return set_err(c, PCSF_EINVAL, "pcsf_forward: unknown precision");
   if (rho_per_region) {
     const size_t nr = (size_t)b->nreg * m->nm;
@@ -803,7 +684,9 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
                      "CamlPaml.P14n.instantiate_tree: domain violation on variable 0");
   HIP_TRY(c, hipSetDevice(c->device));
   float tp = 0.f;
-  int rc = ensure_transitions(m, r, &tp);
+  const int fmt = precision == PCSF_FP64 ? (m->fixed.fmt < 0 ? kFragF32 : m->fixed.fmt)
+                                         : frag_format(precision);
+  int rc = ensure_transitions(m, r, fmt, &tp);
   if (rc) return rc;
   b->t_pbuild = tp;
   b->last_rho.resize((size_t)b->nreg * m->nm);
@@ -815,8 +698,8 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
     e = launch_prune_fp64(m, b, c->stream);
     if (e == hipErrorInvalidValue)
       return set_err(c, PCSF_EINVAL, "FP64 kernel: tree workspace exceeds 64 KiB of LDS");
-  } else if (precision == PCSF_FP32) {
-    e = launch_prune_mfma(m, b, m->mfma_variant, c->stream);
+  } else if (frag_format(precision) >= 0) {
+    e = launch_prune_mfma(m, b, precision, m->mfma_variant[precision], c->stream);
   } else {
     return set_err(c, PCSF_EINVAL, "pcsf_forward: unknown precision");
   }
@@ -837,26 +720,29 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   return PCSF_OK;
 }
 
-int pcsf_autotune(pcsf_batch* b, int reps, double* ms_out, int* chosen) {
+int pcsf_autotune(pcsf_batch* b, int precision, int reps, double* ms_out, int* chosen) {
   if (!b || reps < 1) return PCSF_EINVAL;
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
+  if (precision == PCSF_FP64 || frag_format(precision) < 0)
+    return set_err(c, PCSF_EINVAL, "pcsf_autotune: not an MFMA precision");
   HIP_TRY(c, hipSetDevice(c->device));
-  if (m->cached_rho.empty()) {
+  {
     std::vector<double> one(m->nm, 1.0);
-    int rc = ensure_transitions(m, one, nullptr);
+    int rc = ensure_transitions(m, one, frag_format(precision), nullptr);
     if (rc) return rc;
   }
   hipEvent_t e0, e1;
   HIP_TRY(c, hipEventCreate(&e0));
   HIP_TRY(c, hipEventCreate(&e1));
   double best = 1e300;
-  int best_v = m->mfma_variant;
+  int best_v = m->mfma_variant[precision];
   hipError_t err = hipSuccess;
   for (int v = 0; v < kNumMfmaVariants && err == hipSuccess; ++v) {
-    err = launch_prune_mfma(m, b, v, c->stream);  // warm-up
+    err = launch_prune_mfma(m, b, precision, v, c->stream);  // warm-up
     if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
-    for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch_prune_mfma(m, b, v, c->stream);
+    for (int r = 0; r < reps && err == hipSuccess; ++r)
+      err = launch_prune_mfma(m, b, precision, v, c->stream);
     if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
     if (err == hipSuccess) err = hipEventSynchronize(e1);
     float ms = 0.f;
@@ -871,16 +757,25 @@ int pcsf_autotune(pcsf_batch* b, int reps, double* ms_out, int* chosen) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIP_TRY(c, err);
-  m->mfma_variant = best_v;
+  m->mfma_variant[precision] = best_v;
   if (chosen) *chosen = best_v;
   return PCSF_OK;
 }
 
-int pcsf_set_variant(pcsf_model* m, int variant) {
+int pcsf_model_specialized(pcsf_model* m, int enable, int* available) {
   if (!m) return PCSF_EINVAL;
+  if (enable == 0 || enable == 1) m->specialized = enable == 1;
+  if (available) *available = (m->tree_p ? 1 : 0) | (m->tree_t ? 2 : 0);
+  return PCSF_OK;
+}
+
+int pcsf_set_variant(pcsf_model* m, int precision, int variant) {
+  if (!m) return PCSF_EINVAL;
+  if (precision == PCSF_FP64 || frag_format(precision) < 0)
+    return set_err(m->ctx, PCSF_EINVAL, "pcsf_set_variant: not an MFMA precision");
   if (variant < 0 || variant >= kNumMfmaVariants)
     return set_err(m->ctx, PCSF_EINVAL, "pcsf_set_variant: no such launch variant");
-  m->mfma_variant = variant;
+  m->mfma_variant[precision] = variant;
   return PCSF_OK;
 }
 

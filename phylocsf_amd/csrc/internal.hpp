@@ -7,28 +7,9 @@
 #include <vector>
 
 #include "../../include/phylocsf_hip.h"
+#include "host/program.hpp"
 #include "host/ratematrix.hpp"
 
-// Op program of the fused whole-tree kernel: a post-order walk of the tree
-// that keeps at most `slots` partial-likelihood tiles live in registers.
-// Four words per op: word0 = opcode | depth << 8 | use_prefetch << 16,
-// word1 = arg0 | arg1 << 16, word2 = leaf whose gather a GEMM issues ahead
-// (kNoBranch if none), word3 = 0.  use_prefetch marks a gather op whose
-// first leaf (arg0) was issued by the preceding GEMM.
-enum PcsfOp : int {
-  OP_CHERRY = 1,   // slot[d] = gather(leaf a0) * gather(leaf a1); rescale
-  OP_GEMM = 2,     // slot[d] = P_a0 slot[d] (MFMA, P tile in LDS); a1 = next GEMM's branch
-  OP_LEAFMUL = 3,  // slot[d] *= gather(leaf a0); rescale
-  OP_MULPOP = 4,   // slot[d] *= slot[d+1]; rescale
-  OP_ROOT = 5,     // write ln z and elpr for slot[d]
-  OP_END = 6,
-  OP_CHERRYT = 7,  // slot[d] = message of cherry node w2 above it, from its table at the
-                   // codes of leaves a0, a1 (replaces CHERRY + GEMM; fixed scales only)
-};
-constexpr int kTabCodes = 65;  // table index per leaf: codon 0..63, 64 = `Marginalize
-constexpr int kNoBranch = 0xFFFF;
-constexpr int kOpWords = 4;
-constexpr int kMaxSlots = 6;
 constexpr int kTileCols = 16;   // columns per wave
 constexpr int kGroupCols = 128; // columns per workgroup (8 waves)
 constexpr int kTileFloats = 4096 + 64;  // P tile staged in LDS: fragments + diagonal
@@ -36,8 +17,18 @@ constexpr int kTileFloats = 4096 + 64;  // P tile staged in LDS: fragments + dia
 // Transition-matrix sets on the device: set s is model s % nm at tree scale
 // rho[s] (the fixed strategy uses nm sets; per-region scales use one set per
 // (region, model) of a chunk of regions).
+// Formats of the P tile (PSets::d_Pfrag, kTileFloats floats per branch):
+//   kFragF32   off-diagonal FP32 in v_mfma_f32_16x16x4f32 A order + diagonal
+//   kFragF16   off-diagonal x 2^14 as FP16 hi, then FP16 lo, in
+//              v_mfma_f32_16x16x32_f16 A order (4096 halves each) + diagonal
+//   kFragBF16  the same in BF16, unscaled
+// The diagonal part is always 64 FP32 values P[a][a] - 1 at float 4096.
+constexpr int kFragF32 = 0, kFragF16 = 1, kFragBF16 = 2;
+int frag_format(int precision);  // -1 for a precision without MFMA tiles
+
 struct PSets {
   int cap = 0;                 // sets allocated
+  int fmt = -1;                // format of d_Pfrag as last built
   bool fp64 = false;           // P64T allocated
   double* d_P64T = nullptr;    // cap * nbr * 4096 FP64 P^T (P64T[b][a] = P[a][b])
   float* d_PT32 = nullptr;     // cap * nbr * 4096 FP32 P^T[c][a] = P[a][c] / P[c][c]
@@ -76,7 +67,12 @@ struct pcsf_model {
   int32_t* d_cherries = nullptr;  // node, left leaf, right leaf per table
   int slots = 0;
   int n_gemm = 0;
-  int mfma_variant = 1;  // launch shape of the fused kernel (pcsf_autotune)
+  int mfma_variant[5] = {1, 1, 1, 1, 1};  // launch shape per precision (pcsf_autotune)
+  // tree-specialised programs matching prog / prog_t (nullptr: none), and
+  // whether launches may use them
+  const void* tree_p = nullptr;
+  const void* tree_t = nullptr;
+  bool specialized = true;
   int first_gemm_br = kNoBranch;
   // device: model constants
   double* d_V = nullptr;     // nm * 4096
@@ -91,6 +87,7 @@ struct pcsf_model {
   int32_t* d_prog = nullptr;
   // device: transition matrices at the cached tree scales (set q = model q)
   std::vector<double> cached_rho;
+  int cached_fmt = -1;
   PSets fixed;
   int* d_perr = nullptr;     // error record of the P builder
   double* d_perrval = nullptr;
@@ -138,14 +135,15 @@ struct RegionChunk {
 // ---- kernel launchers (kernels/*.hip) --------------------------------------
 hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uint8_t* dst,
                                   int64_t ld, hipStream_t s);
-hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, hipStream_t s);
+hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int fmt,
+                         hipStream_t s);
 hipError_t launch_cherry_tables(const pcsf_model* m, const PSets& sets, int nsets,
                                 hipStream_t s);
 hipError_t launch_prune_fp64(const pcsf_model* m, const pcsf_batch* b, hipStream_t s);
 hipError_t launch_prune_fp64_regions(const pcsf_model* m, const pcsf_batch* b,
                                      const RegionChunk& ch, hipStream_t s);
-hipError_t launch_prune_mfma_regions(const pcsf_model* m, const pcsf_batch* b, int variant,
-                                     const RegionChunk& ch, hipStream_t s);
+hipError_t launch_prune_mfma_regions(const pcsf_model* m, const pcsf_batch* b, int precision,
+                                     int variant, const RegionChunk& ch, hipStream_t s);
 hipError_t launch_align_codes(const pcsf_batch* b, hipStream_t s);
 hipError_t launch_adjoint_fp64(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
                                double* grad_branch, hipStream_t s);
@@ -160,8 +158,11 @@ struct MfmaVariant {
 };
 constexpr int kNumMfmaVariants = 6;
 extern const MfmaVariant kMfmaVariants[kNumMfmaVariants];
-hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int variant,
-                             hipStream_t s);
+hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int precision,
+                             int variant, hipStream_t s);
+// The generated straight-line program whose op words equal `ops` (built
+// with or without cherry tables), or nullptr.
+const void* find_tree_program(const std::vector<int32_t>& ops, bool tables);
 hipError_t launch_region_reduce(const pcsf_batch* b, hipStream_t s);
 hipError_t launch_lik_columns(int k, int n, const int32_t* d_lc, const int32_t* d_rc,
                               const double* d_P, const double* d_prior, const int32_t* d_leaves,

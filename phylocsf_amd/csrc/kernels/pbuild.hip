@@ -13,9 +13,12 @@
 //   PT32  FP32 leaf messages of the MFMA kernel: PT32[c][a] = P[a][c] / P[c][c],
 //         with lnPcc[c] = ln P[c][c] in FP64 (the dominant entry of a leaf
 //         message is exactly 1 and its value travels in FP64).
-//   Pfrag FP32 tile of kTileFloats: the off-diagonal of P in the A-operand
-//         order of v_mfma_f32_16x16x4f32 as the MFMA kernel consumes it
-//         (diagonal entries zero), followed by the 64 values P[a][a] - 1.
+//   Pfrag tile of kTileFloats floats in format fmt: the off-diagonal of P
+//         in the A-operand order of the MFMA the kernel uses (diagonal
+//         entries zero) -- FP32 for v_mfma_f32_16x16x4f32, or hi and lo
+//         halves (FP16 scaled by 2^14, or BF16) for
+//         v_mfma_f32_16x16x32_{f16,bf16} -- followed by the 64 FP32 values
+//         P[a][a] - 1.
 #include "../internal.hpp"
 
 namespace {
@@ -26,7 +29,7 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
     const double* __restrict__ V, const double* __restrict__ Vinv,
     const double* __restrict__ lam, const double* __restrict__ branch,
     const double* __restrict__ rho, const uint8_t* __restrict__ active, int nm, int nbr,
-    double* __restrict__ P64T, float* __restrict__ PT32, float* __restrict__ Pfrag,
+    double* __restrict__ P64T, float* __restrict__ PT32, float* __restrict__ Pfrag, int fmt,
     double* __restrict__ lnPcc, int* __restrict__ perr, double* __restrict__ perrval) {
   __shared__ double W[64 * 64];  // diag(exp(lam t)) V^-1, then P
   __shared__ double Vs[64 * 64];
@@ -98,15 +101,40 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
   }
   if (tid < 64) lnPcc[((size_t)set * nbr + br) * 64 + tid] = log(Vs[tid * 64 + tid]);
   const size_t fbase = ((size_t)set * nbr + br) * kTileFloats;
-  // fragment order of v_mfma_f32_16x16x4f32's A operand with the K index
-  // permuted to match the accumulator layout (prune_mfma.hip):
-  //   e = ((mo*4 + mb)*64 + lane)*4 + v
-  //   row = 16*mo + lane%16, col = 16*mb + 4*(lane/16) + v
-  for (int e = tid; e < 4096; e += 256) {
-    const int v = e & 3, lane = (e >> 2) & 63, mb = (e >> 8) & 3, mo = e >> 10;
-    const int row = 16 * mo + (lane & 15);
-    const int col = 16 * mb + 4 * (lane >> 4) + v;
-    Pfrag[fbase + e] = (row == col) ? 0.0f : (float)Vs[row * 64 + col];
+  if (fmt == kFragF32) {
+    // fragment order of v_mfma_f32_16x16x4f32's A operand with the K index
+    // permuted to match the accumulator layout (prune_mfma.hpp):
+    //   e = ((mo*4 + mb)*64 + lane)*4 + v
+    //   row = 16*mo + lane%16, col = 16*mb + 4*(lane/16) + v
+    for (int e = tid; e < 4096; e += 256) {
+      const int v = e & 3, lane = (e >> 2) & 63, mb = (e >> 8) & 3, mo = e >> 10;
+      const int row = 16 * mo + (lane & 15);
+      const int col = 16 * mb + 4 * (lane >> 4) + v;
+      Pfrag[fbase + e] = (row == col) ? 0.0f : (float)Vs[row * 64 + col];
+    }
+  } else {
+    // v_mfma_f32_16x16x32_{f16,bf16} A order, K permuted the same way:
+    //   e = ((mo*2 + st)*64 + lane)*8 + j
+    //   row = 16*mo + lane%16, col = 16*(2*st + j/4) + 4*(lane/16) + j%4
+    // hi = x rounded to nearest, lo = the residual x - hi rounded to nearest
+    const double sc = (fmt == kFragF16) ? 16384.0 : 1.0;
+    for (int e = tid; e < 4096; e += 256) {
+      const int j = e & 7, lane = (e >> 3) & 63, st = (e >> 9) & 1, mo = e >> 10;
+      const int row = 16 * mo + (lane & 15);
+      const int col = 16 * (2 * st + (j >> 2)) + 4 * (lane >> 4) + (j & 3);
+      const double x = (row == col) ? 0.0 : Vs[row * 64 + col] * sc;
+      if (fmt == kFragF16) {
+        _Float16* h = reinterpret_cast<_Float16*>(Pfrag + fbase);
+        const _Float16 hi = (_Float16)(float)x;
+        h[e] = hi;
+        h[4096 + e] = (_Float16)(float)(x - (double)(float)hi);
+      } else {
+        __bf16* h = reinterpret_cast<__bf16*>(Pfrag + fbase);
+        const __bf16 hi = (__bf16)(float)x;
+        h[e] = hi;
+        h[4096 + e] = (__bf16)(float)(x - (double)(float)hi);
+      }
+    }
   }
   if (tid < 64) Pfrag[fbase + 4096 + tid] = (float)(Vs[tid * 64 + tid] - 1.0);
 }
@@ -157,12 +185,13 @@ hipError_t launch_cherry_tables(const pcsf_model* m, const PSets& sets, int nset
   return hipGetLastError();
 }
 
-hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, hipStream_t s) {
+hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int fmt,
+                         hipStream_t s) {
   if (nsets <= 0) return hipSuccess;
   dim3 grid(m->nbr, nsets);
   hipLaunchKernelGGL(pbuild_kernel, grid, dim3(256), 0, s, m->d_V, m->d_Vinv, m->d_lam,
                      m->d_branch, sets.d_rho, sets.d_active, m->nm, m->nbr,
-                     sets.fp64 ? sets.d_P64T : nullptr, sets.d_PT32, sets.d_Pfrag,
+                     sets.fp64 ? sets.d_P64T : nullptr, sets.d_PT32, sets.d_Pfrag, fmt,
                      sets.d_lnPcc, m->d_perr, m->d_perrval);
   return hipGetLastError();
 }

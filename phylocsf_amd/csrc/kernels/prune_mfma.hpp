@@ -1,4 +1,6 @@
-// prune_mfma.hip -- fused whole-tree Felsenstein pruning on MFMA (gfx950).
+// prune_mfma.hpp -- fused whole-tree Felsenstein pruning on MFMA (gfx950).
+// The kernel template is instantiated per GEMM precision in its own
+// translation unit (prune_mfma_{f32,f16s,f16,bf16}.hip).
 //
 // Same result as PhyloLik.ensure_alpha + the root posterior
 // (lib/CamlPaml/PhyloLik.ml:73-93,127-138) summed by
@@ -37,11 +39,32 @@
 //    gathers do not wait on a dependent chain of global loads.
 //  * Workgroups are mapped to models by XCD (blockIdx % 8), so each XCD's
 //    L2 holds one model's matrices.
+//  * GEMM precision (template PREC): FP32 on v_mfma_f32_16x16x4f32; or P and
+//    the partial enter v_mfma_f32_16x16x32_{f16,bf16} in half precision with
+//    FP32 accumulation.  FP16 operands are scaled by 2^14 (P's off-diagonal
+//    entries are below 1 and partials are normalised to a maximum below 1, so
+//    nothing overflows and values down to 2^-28 stay normal) and the product
+//    is scaled back by 2^-28, exactly.  The split-FP16 precision carries each
+//    operand as hi + lo (lo = the FP16 rounding of the residual) and forms
+//    hi*hi + hi*lo + lo*hi: about 22 significant bits per product at three
+//    FP16 MFMAs, against sixteen FP32 MFMAs for the same 64x64x16 product.
+//    Live partials stay FP32 in registers in every precision; the diagonal
+//    of P stays in the FP32 VALU term.
+#include <type_traits>
+
 #include "../internal.hpp"
 
-namespace {
+namespace pcsf_mfma {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// precisions of the GEMM (values of PCSF_FP32 .. PCSF_BF16)
+constexpr int kPrecF32 = PCSF_FP32, kPrecF16S = PCSF_FP16S, kPrecF16 = PCSF_FP16,
+              kPrecBF16 = PCSF_BF16;
+constexpr float kF16Scale = 16384.0f;                    // 2^14
+constexpr float kF16Unscale = 1.0f / (16384.0f * 16384.0f);  // 2^-28
 
 struct MfmaParams {
   const uint8_t* codes;
@@ -158,11 +181,267 @@ __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
   E += e;
 }
 
-template <int NS, int WPG, int MINW, bool PREF>
+// 64x64 (off-diagonal P, LDS tile Ab) times the 64x16 tile s in half
+// precision; acc = the product, unscaled.  Tile layout of the halves
+// (pbuild.hip): hi fragments, then lo fragments, 512 x 8 values each, in the
+// A-operand order of v_mfma_f32_16x16x32_{f16,bf16} with K permuted so that
+// K-step st of a lane in group g takes rows 16*(2*st + j/4) + 4*g + j%4 of
+// its column: exactly what the accumulator layout leaves in registers
+// s.r[2*st + j/4][j%4].
+template <int PREC>
+__device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], const Tile& s, const float* Ab,
+                                          int lane) {
+  constexpr bool kBF = PREC == kPrecBF16;
+  using V8 = typename std::conditional<kBF, bf16x8, f16x8>::type;
+  using E = typename std::conditional<kBF, __bf16, _Float16>::type;
+  const float sc = kBF ? 1.0f : kF16Scale;
+  V8 bh[2], bl[2];
+#pragma unroll
+  for (int st = 0; st < 2; ++st)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float x = s.r[2 * st + (j >> 2)][j & 3] * sc;
+      const E h = (E)x;
+      bh[st][j] = h;
+      if constexpr (PREC == kPrecF16S) bl[st][j] = (E)(x - (float)h);
+    }
+  const V8* Ah = reinterpret_cast<const V8*>(Ab);
+  const V8* Al = Ah + 512;
+#pragma unroll
+  for (int mo = 0; mo < 4; ++mo) acc[mo] = (f32x4)(0.0f);
+  auto mma = [](const V8& a, const V8& b, const f32x4& c) -> f32x4 {
+    if constexpr (kBF) return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  };
+  if constexpr (PREC == kPrecF16S) {  // small cross terms first
+#pragma unroll
+    for (int st = 0; st < 2; ++st)
+#pragma unroll
+      for (int mo = 0; mo < 4; ++mo) {
+        const int f = (mo * 2 + st) * 64 + lane;
+        acc[mo] = mma(Al[f], bh[st], acc[mo]);
+        acc[mo] = mma(Ah[f], bl[st], acc[mo]);
+      }
+  }
+#pragma unroll
+  for (int st = 0; st < 2; ++st)
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo) acc[mo] = mma(Ah[(mo * 2 + st) * 64 + lane], bh[st], acc[mo]);
+  if constexpr (!kBF) {
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo) acc[mo] *= kF16Unscale;
+  }
+}
+
+// One wave's walk of the op program over its 16 columns, and the ops.  The
+// interpreter (Interp) and the tree-specialised straight-line programs
+// (generated per shipped tree at build time) both drive these members, so
+// the arithmetic is the same code in both.
+template <int PREC, int WPG, bool PREF>
+struct Walk {
+  static constexpr int NT = 64 * WPG;        // threads
+  static constexpr int PER = 4096 / NT / 4;  // float4 of the fragments per thread
+  static constexpr int GC = 16 * WPG;        // columns per workgroup
+  const MfmaParams& p;
+  float (*Pbuf)[kTileFloats];  // LDS: double-buffered P tile
+  const uint8_t* codes_s;      // LDS: the group's leaf codes, nl x GC
+  int tid, lane, grp, ccol;
+  int q = 0, set = 0;          // model and transition set of the current task
+  int64_t col = 0;             // this lane's column
+  bool has_next = false;       // a next task exists (its first tile is prefetched)
+  int next_set = 0;
+  int kg = 0;                  // GEMMs done (selects the LDS buffer)
+  int E = 0;                   // sum of the power-of-two rescaling exponents
+  double LS = 0.0;             // sum of ln P[c][c] of the gathered leaves
+  Gathered PF;                 // leaf gather issued ahead by the previous GEMM
+
+  __device__ __forceinline__ Walk(const MfmaParams& prm, float (*pb)[kTileFloats],
+                                  const uint8_t* cs, int t, int l, int g, int cc)
+      : p(prm), Pbuf(pb), codes_s(cs), tid(t), lane(l), grp(g), ccol(cc) {
+    PF.marg = true;
+    PF.ln = 0.0;
+  }
+
+  // The transition set as an opaque scalar: each op recomputes its table
+  // addresses from it instead of the compiler keeping every op's address
+  // live across a straight-line program (which overflows the SGPRs).
+  __device__ __forceinline__ int oset() const {
+    int s = set;
+    asm volatile("" : "+s"(s));
+    return s;
+  }
+  __device__ __forceinline__ Gathered& first_leaf(int leaf, bool use_pf) {
+    if (!(PREF && use_pf)) gather(PF, p, codes_s, GC, oset(), leaf, ccol, grp);
+    return PF;
+  }
+  // s = leaf a0 o leaf a1 (rescaled)
+  __device__ __forceinline__ void cherry(Tile& s, uint64_t& k, int a0, int a1, bool use_pf) {
+    Gathered g1;
+    gather(g1, p, codes_s, GC, oset(), a1, ccol, grp);
+    Gathered& g0 = first_leaf(a0, use_pf);
+    s = leaf_tile(g0, LS);
+    k = __ballot(g0.marg && g1.marg);
+    mul_rescale(s, leaf_tile(g1, LS), E);
+  }
+  // s = message above the cherry (a0, a1), from table `tab`
+  __device__ __forceinline__ void cherryt(Tile& s, uint64_t& k, int a0, int a1, int tab) {
+    const int ca = codes_s[a0 * GC + ccol], cb = codes_s[a1 * GC + ccol];
+    const bool ma = ca == PCSF_MARGINALIZE, mb = cb == PCSF_MARGINALIZE;
+    const size_t e = ((size_t)oset() * p.nctab + tab) * (kTabCodes * kTabCodes) +
+                     (ma ? 64 : ca) * kTabCodes + (mb ? 64 : cb);
+    const float* src = p.ctab + e * 64 + 4 * grp;
+#pragma unroll
+    for (int mbk = 0; mbk < 4; ++mbk) s.r[mbk] = *reinterpret_cast<const f32x4*>(src + 16 * mbk);
+    LS += p.clnt[e];
+    k = __ballot(ma && mb);
+  }
+  // s = s o leaf a0 (rescaled)
+  __device__ __forceinline__ void leafmul(Tile& s, uint64_t& k, int a0, bool use_pf) {
+    Gathered& g0 = first_leaf(a0, use_pf);
+    k &= __ballot(g0.marg);
+    mul_rescale(s, leaf_tile(g0, LS), E);
+  }
+  // s = s o o (rescaled)
+  __device__ __forceinline__ void mulpop(Tile& s, uint64_t& k, const Tile& o, uint64_t ko) {
+    k &= ko;
+    mul_rescale(s, o, E);
+  }
+  // s = P s for the P tile staged in LDS; prefetches the tile of branch
+  // next_br (or the next task's first tile) and, if pf_leaf names one, the
+  // next leaf gather
+  __device__ __forceinline__ void gemm(Tile& s, uint64_t k, int next_br, int pf_leaf) {
+    __syncthreads();  // Pbuf[kg&1] complete; Pbuf[(kg&1)^1] no longer read
+    const int st = oset();
+    if (PREF && pf_leaf != kNoBranch) gather(PF, p, codes_s, GC, st, pf_leaf, ccol, grp);
+    const float* nsrc = nullptr;
+    if (next_br != kNoBranch) nsrc = p.Pfrag + ((size_t)st * p.nbr + next_br) * kTileFloats;
+    else if (has_next)
+      nsrc = p.Pfrag + ((size_t)next_set * p.nbr + p.first_gemm_br) * kTileFloats;
+    f32x4 pf[PER + 1];
+    if (nsrc) {
+#pragma unroll
+      for (int r = 0; r < PER; ++r)
+        pf[r] = *reinterpret_cast<const f32x4*>(nsrc + (tid + NT * r) * 4);
+      if (tid < 16) pf[PER] = *reinterpret_cast<const f32x4*>(nsrc + 4096 + tid * 4);
+    }
+    const float* Ab = Pbuf[kg & 1];
+    f32x4 acc[4];
+    if constexpr (PREC == kPrecF32) {
+#pragma unroll
+      for (int mo = 0; mo < 4; ++mo) acc[mo] = (f32x4)(0.0f);
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        f32x4 A[4];
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo)
+          A[mo] = *reinterpret_cast<const f32x4*>(Ab + ((mo * 4 + mb) * 64 + lane) * 4);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float bv = s.r[mb][v];
+#pragma unroll
+          for (int mo = 0; mo < 4; ++mo)
+            acc[mo] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mo][v], bv, acc[mo], 0, 0, 0);
+        }
+      }
+    } else {
+      gemm_half<PREC>(acc, s, Ab, lane);
+    }
+    // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, in round-to-nearest VALU
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo) {
+      const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[mo][v] = s.r[mo][v] + fmaf(dd[v], s.r[mo][v], acc[mo][v]);
+    }
+    const bool cst = (k >> lane) & 1;
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo) s.r[mo] = cst ? s.r[mo] : acc[mo];
+    if (nsrc) {
+      float* dst = Pbuf[(kg & 1) ^ 1];
+#pragma unroll
+      for (int r = 0; r < PER; ++r) *reinterpret_cast<f32x4*>(&dst[(tid + NT * r) * 4]) = pf[r];
+      if (tid < 16) *reinterpret_cast<f32x4*>(&dst[4096 + tid * 4]) = pf[PER];
+    }
+    ++kg;
+  }
+  // ln z and elpr of this lane's column from the root tile
+  __device__ __forceinline__ void root(const Tile& s) {
+    const double* piq = p.pi + q * 64;
+    const double* lnq = p.lnpi + q * 64;
+    double z = 0.0, el = 0.0;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * mb + 4 * grp + v;
+        const double x = (double)s.r[mb][v] * piq[row];
+        z += x;
+        el = fma(x, lnq[row], el);
+      }
+    z += __shfl_xor(z, 16);
+    el += __shfl_xor(el, 16);
+    z += __shfl_xor(z, 32);
+    el += __shfl_xor(el, 32);
+    if (grp == 0) {
+      p.logz[(size_t)q * p.ld + col] = log(z) + (double)E * 0.69314718055994530942 + LS;
+      p.colelpr[(size_t)q * p.ld + col] = (z == 0.0) ? 0.0 : el / z;
+    }
+  }
+};
+
+// The op-program interpreter: reads the host's program (four words per op)
+// and dispatches on (opcode, slot).
+template <int NS>
+struct Interp {
+  template <class W>
+  __device__ __forceinline__ static void run(W& w, Tile (&S)[NS], uint64_t (&K)[NS]) {
+    const int32_t* prog = w.p.prog;
+    for (int pc = 0;; ++pc) {
+      const int w0 = __builtin_amdgcn_readfirstlane(prog[4 * pc]);
+      const int w1 = __builtin_amdgcn_readfirstlane(prog[4 * pc + 1]);
+      const int w2 = __builtin_amdgcn_readfirstlane(prog[4 * pc + 2]);
+      const int op = w0 & 0xff, d = (w0 >> 8) & 0xff;
+      const bool use_pf = (w0 >> 16) & 1;
+      const int a0 = w1 & 0xffff, a1 = (w1 >> 16) & 0xffff;
+      if (op == OP_END) break;
+#define PCSF_CASES(D)                                          \
+  case OP_CHERRY * 8 + D:                                      \
+    if constexpr (D < NS) w.cherry(S[D], K[D], a0, a1, use_pf); \
+    break;                                                     \
+  case OP_GEMM * 8 + D:                                        \
+    if constexpr (D < NS) w.gemm(S[D], K[D], a1, w2 & 0xffff); \
+    break;                                                     \
+  case OP_LEAFMUL * 8 + D:                                     \
+    if constexpr (D < NS) w.leafmul(S[D], K[D], a0, use_pf);    \
+    break;                                                     \
+  case OP_CHERRYT * 8 + D:                                     \
+    if constexpr (D < NS) w.cherryt(S[D], K[D], a0, a1, w2);   \
+    break;                                                     \
+  case OP_MULPOP * 8 + D:                                      \
+    if constexpr (D + 1 < NS) w.mulpop(S[D], K[D], S[D + 1], K[D + 1]); \
+    break;
+      switch (op * 8 + d) {
+        PCSF_CASES(0)
+        PCSF_CASES(1)
+        PCSF_CASES(2)
+        PCSF_CASES(3)
+        PCSF_CASES(4)
+        PCSF_CASES(5)
+        case OP_ROOT * 8 + 0:
+          w.root(S[0]);
+          break;
+        default:
+          break;
+      }
+#undef PCSF_CASES
+    }
+  }
+};
+
+template <int PREC, int NS, int WPG, int MINW, bool PREF, class PROG>
 __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p) {
-  constexpr int NT = 64 * WPG;          // threads
-  constexpr int PER = 4096 / NT / 4;    // float4 of the fragments per thread
-  constexpr int GC = 16 * WPG;          // columns per workgroup
+  using W = Walk<PREC, WPG, PREF>;
+  constexpr int NT = W::NT, PER = W::PER, GC = W::GC;
   static_assert(NT >= 16, "the diagonal (16 float4) is copied by the first 16 threads");
   __shared__ __attribute__((aligned(16))) float Pbuf[2][kTileFloats];
   extern __shared__ __attribute__((aligned(16))) uint8_t codes_s[];  // nl x GC
@@ -181,7 +460,6 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
   Task cur;
   int64_t jc = 0;
   if (!next_task<GC>(p, jc, &cur)) return;
-  int kg = 0;
   if (p.first_gemm_br != kNoBranch) {
     const float* src = p.Pfrag + ((size_t)cur.set * p.nbr + p.first_gemm_br) * kTileFloats;
 #pragma unroll
@@ -193,17 +471,13 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
           *reinterpret_cast<const f32x4*>(src + 4096 + tid * 4);
   }
 
-  Gathered PF;  // leaf gather issued ahead by the previous GEMM
-  PF.marg = true;
-  PF.ln = 0.0;
   const int ccol = wave * 16 + lcol;  // column within the workgroup's group
+  W w(p, Pbuf, codes_s, tid, lane, grp, ccol);
   for (;;) {
     Task nxt;
     int64_t jn = jc + 1;
     const bool has_next = next_task<GC>(p, jn, &nxt);
-    const int q = cur.q, set = cur.set;
     const int64_t gabs = p.g0 + cur.g;
-    const int64_t col = gabs * GC + ccol;
     // stage this group's leaf codes (nl x GC bytes) in LDS
     __syncthreads();  // the previous task's gathers are done with codes_s
     {
@@ -211,169 +485,28 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
       uint32_t* dst = reinterpret_cast<uint32_t*>(codes_s);
       const int words = GC / 4;
       for (int e = tid; e < p.nl * words; e += NT) {
-        const int l = e / words, w = e - l * words;
-        dst[e] = src[((size_t)l * p.ld + (size_t)gabs * GC) / 4 + w];
+        const int l = e / words, wd = e - l * words;
+        dst[e] = src[((size_t)l * p.ld + (size_t)gabs * GC) / 4 + wd];
       }
     }
     __syncthreads();
-    int E = 0;
-    double LS = 0.0;  // sum of ln P[c][c] of the gathered leaves
-    for (int pc = 0;; ++pc) {
-      const int w0 = __builtin_amdgcn_readfirstlane(p.prog[4 * pc]);
-      const int w1 = __builtin_amdgcn_readfirstlane(p.prog[4 * pc + 1]);
-      const int w2 = __builtin_amdgcn_readfirstlane(p.prog[4 * pc + 2]);
-      const int op = w0 & 0xff, d = (w0 >> 8) & 0xff, use_pf = PREF ? (w0 >> 16) & 1 : 0;
-      const int a0 = w1 & 0xffff, a1 = (w1 >> 16) & 0xffff;
-      if (op == OP_END) break;
-
-      auto first_leaf = [&](int leaf) -> Gathered& {
-        if (!use_pf) gather(PF, p, codes_s, GC, set, leaf, ccol, grp);
-        return PF;
-      };
-      auto do_cherry = [&](Tile& s, uint64_t& k) {
-        Gathered g1;
-        gather(g1, p, codes_s, GC, set, a1, ccol, grp);
-        Gathered& g0 = first_leaf(a0);
-        s = leaf_tile(g0, LS);
-        k = __ballot(g0.marg && g1.marg);
-        mul_rescale(s, leaf_tile(g1, LS), E);
-      };
-      auto do_cherryt = [&](Tile& s, uint64_t& k) {
-        const int ca = codes_s[a0 * GC + ccol], cb = codes_s[a1 * GC + ccol];
-        const bool ma = ca == PCSF_MARGINALIZE, mb = cb == PCSF_MARGINALIZE;
-        const size_t e = ((size_t)set * p.nctab + w2) * (kTabCodes * kTabCodes) +
-                         (ma ? 64 : ca) * kTabCodes + (mb ? 64 : cb);
-        const float* src = p.ctab + e * 64 + 4 * grp;
-#pragma unroll
-        for (int mbk = 0; mbk < 4; ++mbk) s.r[mbk] = *reinterpret_cast<const f32x4*>(src + 16 * mbk);
-        LS += p.clnt[e];
-        k = __ballot(ma && mb);
-      };
-      auto do_leafmul = [&](Tile& s, uint64_t& k) {
-        Gathered& g0 = first_leaf(a0);
-        k &= __ballot(g0.marg);
-        mul_rescale(s, leaf_tile(g0, LS), E);
-      };
-      auto do_gemm = [&](Tile& s, uint64_t k) {
-        __syncthreads();  // Pbuf[kg&1] complete; Pbuf[(kg&1)^1] no longer read
-        if (PREF && (w2 & 0xffff) != kNoBranch)
-          gather(PF, p, codes_s, GC, set, w2 & 0xffff, ccol, grp);
-        const float* nsrc = nullptr;
-        if (a1 != kNoBranch) nsrc = p.Pfrag + ((size_t)set * p.nbr + a1) * kTileFloats;
-        else if (has_next)
-          nsrc = p.Pfrag + ((size_t)nxt.set * p.nbr + p.first_gemm_br) * kTileFloats;
-        f32x4 pf[PER + 1];
-        if (nsrc) {
-#pragma unroll
-          for (int r = 0; r < PER; ++r)
-            pf[r] = *reinterpret_cast<const f32x4*>(nsrc + (tid + NT * r) * 4);
-          if (tid < 16) pf[PER] = *reinterpret_cast<const f32x4*>(nsrc + 4096 + tid * 4);
-        }
-        const float* Ab = Pbuf[kg & 1];
-        f32x4 acc[4];
-#pragma unroll
-        for (int mo = 0; mo < 4; ++mo) acc[mo] = (f32x4)(0.0f);
-#pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-          f32x4 A[4];
-#pragma unroll
-          for (int mo = 0; mo < 4; ++mo)
-            A[mo] = *reinterpret_cast<const f32x4*>(Ab + ((mo * 4 + mb) * 64 + lane) * 4);
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const float bv = s.r[mb][v];
-#pragma unroll
-            for (int mo = 0; mo < 4; ++mo)
-              acc[mo] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mo][v], bv, acc[mo], 0, 0, 0);
-          }
-        }
-        // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, in round-to-nearest VALU
-#pragma unroll
-        for (int mo = 0; mo < 4; ++mo) {
-          const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
-#pragma unroll
-          for (int v = 0; v < 4; ++v)
-            acc[mo][v] = s.r[mo][v] + fmaf(dd[v], s.r[mo][v], acc[mo][v]);
-        }
-        const bool cst = (k >> lane) & 1;
-#pragma unroll
-        for (int mo = 0; mo < 4; ++mo) s.r[mo] = cst ? s.r[mo] : acc[mo];
-        if (nsrc) {
-          float* dst = Pbuf[(kg & 1) ^ 1];
-#pragma unroll
-          for (int r = 0; r < PER; ++r)
-            *reinterpret_cast<f32x4*>(&dst[(tid + NT * r) * 4]) = pf[r];
-          if (tid < 16) *reinterpret_cast<f32x4*>(&dst[4096 + tid * 4]) = pf[PER];
-        }
-        ++kg;
-      };
-      auto do_root = [&](const Tile& s) {
-        const double* piq = p.pi + q * 64;
-        const double* lnq = p.lnpi + q * 64;
-        double z = 0.0, el = 0.0;
-#pragma unroll
-        for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int row = 16 * mb + 4 * grp + v;
-            const double x = (double)s.r[mb][v] * piq[row];
-            z += x;
-            el = fma(x, lnq[row], el);
-          }
-        z += __shfl_xor(z, 16);
-        el += __shfl_xor(el, 16);
-        z += __shfl_xor(z, 32);
-        el += __shfl_xor(el, 32);
-        if (grp == 0) {
-          p.logz[(size_t)q * p.ld + col] = log(z) + (double)E * 0.69314718055994530942 + LS;
-          p.colelpr[(size_t)q * p.ld + col] = (z == 0.0) ? 0.0 : el / z;
-        }
-      };
-
-#define PCSF_CASES(D)                                  \
-  case OP_CHERRY * 8 + D:                              \
-    if constexpr (D < NS) do_cherry(S[D], K[D]);       \
-    break;                                             \
-  case OP_GEMM * 8 + D:                                \
-    if constexpr (D < NS) do_gemm(S[D], K[D]);         \
-    break;                                             \
-  case OP_LEAFMUL * 8 + D:                             \
-    if constexpr (D < NS) do_leafmul(S[D], K[D]);      \
-    break;                                             \
-  case OP_CHERRYT * 8 + D:                             \
-    if constexpr (D < NS) do_cherryt(S[D], K[D]);      \
-    break;                                             \
-  case OP_MULPOP * 8 + D:                              \
-    if constexpr (D + 1 < NS) {                        \
-      K[D] &= K[D + 1];                                \
-      mul_rescale(S[D], S[D + 1], E);                  \
-    }                                                  \
-    break;
-
-      switch (op * 8 + d) {
-        PCSF_CASES(0)
-        PCSF_CASES(1)
-        PCSF_CASES(2)
-        PCSF_CASES(3)
-        PCSF_CASES(4)
-        PCSF_CASES(5)
-        case OP_ROOT * 8 + 0:
-          do_root(S[0]);
-          break;
-        default:
-          break;
-      }
-#undef PCSF_CASES
-    }
+    w.q = cur.q;
+    w.set = cur.set;
+    w.col = gabs * GC + ccol;
+    w.has_next = has_next;
+    w.next_set = has_next ? nxt.set : 0;
+    w.E = 0;
+    w.LS = 0.0;
+    PROG::run(w, S, K);
     if (!has_next) break;
     cur = nxt;
     jc = jn;
   }
 }
 
-template <int NS, int WPG, int MINW, bool PREF>
+template <int PREC, int NS, int WPG, int MINW, bool PREF, class PROG = Interp<NS>>
 hipError_t launch_ns(const MfmaParams& prm, int grid_per_cu, int num_cus, hipStream_t s) {
-  auto kern = prune_mfma_kernel<NS, WPG, MINW, PREF>;
+  auto kern = prune_mfma_kernel<PREC, NS, WPG, MINW, PREF, PROG>;
   const size_t dyn = ((size_t)prm.nl * 16 * WPG + 15) / 16 * 16;  // staged leaf codes
   if (grid_per_cu <= 0) {
     int per_cu = 0;
@@ -386,93 +519,39 @@ hipError_t launch_ns(const MfmaParams& prm, int grid_per_cu, int num_cus, hipStr
   return hipGetLastError();
 }
 
-template <int WPG, int MINW, bool PREF>
+template <int PREC, int WPG, int MINW, bool PREF>
 hipError_t launch_v(const MfmaParams& prm, int slots, int gpc, int cus, hipStream_t s) {
   switch (slots) {
     case 1:
-    case 2: return launch_ns<2, WPG, MINW, PREF>(prm, gpc, cus, s);
-    case 3: return launch_ns<3, WPG, MINW, PREF>(prm, gpc, cus, s);
-    case 4: return launch_ns<4, WPG, MINW, PREF>(prm, gpc, cus, s);
-    case 5: return launch_ns<5, WPG, MINW, PREF>(prm, gpc, cus, s);
-    case 6: return launch_ns<6, WPG, MINW, PREF>(prm, gpc, cus, s);
+    case 2: return launch_ns<PREC, 2, WPG, MINW, PREF>(prm, gpc, cus, s);
+    case 3: return launch_ns<PREC, 3, WPG, MINW, PREF>(prm, gpc, cus, s);
+    case 4: return launch_ns<PREC, 4, WPG, MINW, PREF>(prm, gpc, cus, s);
+    case 5: return launch_ns<PREC, 5, WPG, MINW, PREF>(prm, gpc, cus, s);
+    case 6: return launch_ns<PREC, 6, WPG, MINW, PREF>(prm, gpc, cus, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-}  // namespace
-
-const MfmaVariant kMfmaVariants[kNumMfmaVariants] = {
-    {8, 1, true, "8 waves/WG, gather prefetch"},
-    {4, 2, true, "4 waves/WG, >=2 waves/SIMD, gather prefetch"},
-    {4, 3, false, "4 waves/WG, >=3 waves/SIMD"},
-    {2, 2, true, "2 waves/WG, >=2 waves/SIMD, gather prefetch"},
-    {4, 2, false, "4 waves/WG, >=2 waves/SIMD"},
-    {2, 3, false, "2 waves/WG, >=3 waves/SIMD"},
-};
-
-namespace {
+// Launch shape `variant` (kMfmaVariants) of the precision-PREC kernel; each
+// precision is instantiated in its own translation unit.
+template <int PREC>
 hipError_t launch_variant(const MfmaParams& prm, int variant, int slots, int cus,
                           hipStream_t s) {
   switch (variant) {
-    case 0: return launch_v<8, 1, true>(prm, slots, 0, cus, s);
-    case 1: return launch_v<4, 2, true>(prm, slots, 0, cus, s);
-    case 2: return launch_v<4, 3, false>(prm, slots, 0, cus, s);
-    case 3: return launch_v<2, 2, true>(prm, slots, 0, cus, s);
-    case 4: return launch_v<4, 2, false>(prm, slots, 0, cus, s);
-    case 5: return launch_v<2, 3, false>(prm, slots, 0, cus, s);
+    case 0: return launch_v<PREC, 8, 1, true>(prm, slots, 0, cus, s);
+    case 1: return launch_v<PREC, 4, 2, true>(prm, slots, 0, cus, s);
+    case 2: return launch_v<PREC, 4, 3, false>(prm, slots, 0, cus, s);
+    case 3: return launch_v<PREC, 2, 2, true>(prm, slots, 0, cus, s);
+    case 4: return launch_v<PREC, 4, 2, false>(prm, slots, 0, cus, s);
+    case 5: return launch_v<PREC, 2, 3, false>(prm, slots, 0, cus, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-MfmaParams base_params(const pcsf_model* m, const PSets& sets) {
-  MfmaParams prm{};
-  prm.nl = m->nl;
-  prm.nm = m->nm;
-  prm.nbr = m->nbr;
-  const bool tables = sets.d_ctab != nullptr && !m->cherries.empty();
-  prm.first_gemm_br = tables ? m->first_gemm_br_t : m->first_gemm_br;
-  prm.prog = tables ? m->d_prog_t : m->d_prog;
-  prm.ctab = tables ? sets.d_ctab : nullptr;
-  prm.clnt = tables ? sets.d_clnt : nullptr;
-  prm.nctab = (int)m->cherries.size() / 3;
-  prm.Pfrag = sets.d_Pfrag;
-  prm.PT32 = sets.d_PT32;
-  prm.lnPcc = sets.d_lnPcc;
-  prm.pi = m->d_pi;
-  prm.lnpi = m->d_lnpi;
-  return prm;
-}
-}  // namespace
+extern template hipError_t launch_variant<kPrecF32>(const MfmaParams&, int, int, int, hipStream_t);
+extern template hipError_t launch_variant<kPrecF16S>(const MfmaParams&, int, int, int, hipStream_t);
+extern template hipError_t launch_variant<kPrecF16>(const MfmaParams&, int, int, int, hipStream_t);
+extern template hipError_t launch_variant<kPrecBF16>(const MfmaParams&, int, int, int, hipStream_t);
 
-hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int variant,
-                             hipStream_t s) {
-  if (variant < 0 || variant >= kNumMfmaVariants) return hipErrorInvalidValue;
-  const int wpg = kMfmaVariants[variant].wpg;
-  MfmaParams prm = base_params(m, m->fixed);
-  prm.codes = b->d_codes;
-  prm.ld = b->ld;
-  prm.ngroups = b->ld / (16 * wpg);
-  prm.logz = b->d_logz;
-  prm.colelpr = b->d_colelpr;
-  if (prm.ngroups == 0) return hipSuccess;
-  return launch_variant(prm, variant, m->slots, m->ctx->num_cus, s);
-}
+}  // namespace pcsf_mfma
 
-hipError_t launch_prune_mfma_regions(const pcsf_model* m, const pcsf_batch* b, int variant,
-                                     const RegionChunk& ch, hipStream_t s) {
-  if (variant < 0 || variant >= kNumMfmaVariants) return hipErrorInvalidValue;
-  const int gc = 16 * kMfmaVariants[variant].wpg;
-  if (gc > kAlignCols) return hipErrorInvalidValue;  // a group must not span regions
-  MfmaParams prm = base_params(m, *ch.sets);
-  prm.codes = b->al.d_codes;
-  prm.ld = b->al.ld;
-  prm.g0 = ch.block0 * (kAlignCols / gc);
-  prm.ngroups = ch.nblocks * (kAlignCols / gc);
-  prm.logz = b->d_alogz;
-  prm.colelpr = b->d_acolelpr;
-  prm.block_region = b->al.d_block_region;
-  prm.active = ch.sets->d_active;
-  prm.r0 = ch.r0;
-  if (prm.ngroups == 0) return hipSuccess;
-  return launch_variant(prm, variant, m->slots, m->ctx->num_cus, s);
-}
