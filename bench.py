@@ -137,12 +137,18 @@ def main():
     ap.add_argument("--regions", type=int, default=None, help="override region count")
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fp32",
                     help="GEMM precision of the batched MFMA path")
+    ap.add_argument("--workload", choices=["C2", "C3", "C4", "C5"], default="C2",
+                    help="C2 (default) is the headline measurement; C3/C4/C5 print their own "
+                         "line for the other scoring configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (regions shard "
               "across GPUs by design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
+    if args.workload != "C2":
+        other_workload(args)
+        return
 
     import phylocsf_amd as P
     from phylocsf_amd import workloads as W
@@ -254,6 +260,159 @@ def main():
     }
     if rank == 0:
         print(json.dumps(out), flush=True)
+
+
+def _line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def other_workload(args):
+    """The other scoring configurations, one JSON line each (not the headline):
+      C3  100vertebrates, 2,000 regions, mle: the reference's maximiser and the
+          gradient maximiser (adjoint d lpr/d rho), end-to-end scored columns/s
+          and likelihood evaluations/s;
+      C4  120mammals, 100,000 regions, 6 reading frames built on the GPU from
+          nucleotide rows, FP32 (autotuned) fixed scoring of every frame;
+      C5  100vertebrates, 20,000 regions, every GEMM precision against FP64
+          (error per precision), and per-branch gradients at the mle estimates.
+    """
+    import phylocsf_amd as P
+    from phylocsf_amd import workloads as W
+
+    ctx = P.Context(0)
+    ctx.set_timing(True)
+    name = args.workload
+    model = W.model_for(name, ctx)
+    prec, dtype, _, _ = PRECISIONS[args.precision]
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    if name == "C4":
+        lengths, rmodel, rrho, kw = W.region_params("C4", args.regions)
+        nuc, noff = model.simulate_nucleotides(lengths, rmodel, rrho, seed=W.SPECS["C4"]["seed"],
+                                               n_threads=16, **kw)
+        log("simulated %d regions, %d nucleotides in %.1f s" % (len(lengths), noff[-1],
+                                                                 time.perf_counter() - t0))
+        t1 = time.perf_counter()
+        batch = P.Batch.frames(model, nuc, noff, 6)
+        build_s = time.perf_counter() - t1
+        del nuc
+        log("6-frame batch: %d regions, %d codon columns, built in %.2f s (upload included)"
+            % (batch.n_regions, batch.n_cols, build_s))
+        batch.forward(1.0, prec)
+        chosen, shape_ms = batch.autotune(reps=1, precision=prec)
+        for _ in range(args.warmup):
+            batch.forward(1.0, prec)
+        ctx.synchronize()
+        kms = []
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            ll, el = batch.forward(1.0, prec)
+            kms.append(batch.timing()["prune_ms"])
+        ctx.synchronize()
+        el_s = time.perf_counter() - t1
+        score = P.DB * (ll[:, 0] - ll[:, 1])
+        best_frame = score.reshape(-1, 6).argmax(axis=1)
+        _line("codon-columns/s, 6-frame fixed scoring (FP32 MFMA)", batch.n_cols * steps / el_s,
+              "codon-columns/s", steps, args.warmup, 1000 * el_s / steps, dtype,
+              {"workload": "120mammals, %d regions U{20..600}, 30/70 coding/noncoding, gap runs "
+                           "(mean 5 codons) covering 0.25, 6 reading frames, fixed strategy (seed 4)"
+                           % len(lengths), "codon_columns": batch.n_cols},
+              kernel_ms=float(np.mean(kms)), launch_shape=int(chosen),
+              autotune_ms=[round(float(t), 3) for t in shape_ms],
+              frame_batch_build_s=build_s,
+              best_frame_histogram=np.bincount(best_frame, minlength=6).tolist())
+        return
+    wl = W.make(name, model, n_regions=args.regions, n_threads=16)
+    log("simulated %d regions, %d columns in %.1f s" % (wl.n_regions, wl.n_cols,
+                                                         time.perf_counter() - t0))
+    batch = P.Batch(model, wl.codes, wl.offsets)
+    if name == "C3":
+        res = {}
+        for strat in ("mle", "mle_grad"):
+            P.score(batch, strat, prec)  # warm-up (builds the aligned layout)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            r = P.score(batch, strat, prec)
+            dt = time.perf_counter() - t1
+            res[strat] = (r, dt)
+            log("%s: %.2f s, %.3e scored col/s, %d evaluations per region (mean)"
+                % (strat, dt, wl.n_cols / dt, r["n_evals"].mean()))
+        r64 = P.score(batch, "mle", P.PCSF_FP64)
+        g, dtg = res["mle_grad"]
+        b, dtb = res["mle"]
+        # finite-difference check of d lpr / d rho at the gradient estimates (FP64, step 1e-5)
+        rho = g["rho"]
+        batch.forward(rho, P.PCSF_FP64, per_region=True)
+        d, _ = batch.backward(np.ones_like(rho))
+        lp, _ = batch.forward(rho + 1e-5, P.PCSF_FP64, per_region=True)
+        lm, _ = batch.forward(rho - 1e-5, P.PCSF_FP64, per_region=True)
+        fd = (lp - lm) / 2e-5
+        fd_err = float(np.max(np.abs(d - fd) / np.maximum(1.0, np.abs(fd))))
+        _line("scored codon-columns/s, mle with the adjoint (gradient) maximiser",
+              wl.n_cols / dtg, "codon-columns/s", 1, 1, 1000 * dtg, dtype,
+              {"workload": "100vertebrates, %d regions lognormal(300, 0.8) clipped 30..5000, "
+                           "coding, tree scale U(0.3, 3.0), mle (seed 3)" % wl.n_regions,
+               "codon_columns": wl.n_cols},
+              gradient_evaluations_per_s=float(wl.n_cols * g["n_evals"].mean() / dtg),
+              reference_maximiser={"scored_columns_per_s": wl.n_cols / dtb,
+                                   "evaluations_per_s": float(wl.n_cols * b["n_evals"].mean() / dtb),
+                                   "mean_evaluations": float(b["n_evals"].mean())},
+              mean_gradient_evaluations=float(g["n_evals"].mean()),
+              max_abs_delta_score_db={"mle_grad_vs_fp64_mle": float(np.max(np.abs(g["score"] - r64["score"]))),
+                                      "mle_vs_fp64_mle": float(np.max(np.abs(b["score"] - r64["score"])))},
+              lpr_gain_of_gradient_maximiser=float(np.max(g["lpr"] - r64["lpr"])),
+              fd_rel_error_of_d_lpr_d_rho=fd_err)
+        return
+    # C5: every precision against FP64, fixed strategy; per-branch gradients at the mle
+    ll64, _ = batch.forward(1.0, P.PCSF_FP64)
+    s64 = P.DB * (ll64[:, 0] - ll64[:, 1])
+    per = {}
+    for pname in ("fp32", "fp16s", "fp16", "bf16"):
+        pc = PRECISIONS[pname][0]
+        batch.forward(1.0, pc)
+        batch.autotune(reps=1, precision=pc)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            ll, _ = batch.forward(1.0, pc)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t1) / steps
+        s = P.DB * (ll[:, 0] - ll[:, 1])
+        rel = float(np.max(np.abs(ll - ll64) / np.maximum(1.0, np.abs(ll64))))
+        per[pname] = {"codon_columns_per_s": wl.n_cols / dt,
+                      "max_abs_delta_score_db": float(np.max(np.abs(s - s64))),
+                      "mean_abs_delta_score_db": float(np.mean(np.abs(s - s64))),
+                      "max_rel_loglik_error": rel}
+        log("%s: %.3e col/s, max |d score| %.3g dB, max rel lpr %.3g"
+            % (pname, wl.n_cols / dt, per[pname]["max_abs_delta_score_db"], rel))
+    # gradient maximiser and per-branch gradients (FP64 outside pass) on the
+    # first regions of the workload (about 1.5 M columns)
+    nsub = min(wl.n_regions, 1000)
+    sub = P.Batch(model, wl.codes[:wl.offsets[nsub]], wl.offsets[:nsub + 1])
+    sub_cols = int(wl.offsets[nsub])
+    t1 = time.perf_counter()
+    g = P.score(sub, "mle_grad", prec)
+    dt_mle = time.perf_counter() - t1
+    sub.forward(g["rho"], P.PCSF_FP64, per_region=True)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    _, gbr = sub.backward(np.ones_like(g["rho"]), branches=True)
+    dt_b = time.perf_counter() - t1
+    best = per[args.precision]
+    _line("codon-columns/s per GEMM precision, fixed scoring", best["codon_columns_per_s"],
+          "codon-columns/s", steps, 1, 1000 * wl.n_cols / best["codon_columns_per_s"], dtype,
+          {"workload": "100vertebrates, %d regions U{100..3000}, 50/50 coding/noncoding, gap p=0.15"
+                       " (seed 5)" % wl.n_regions, "codon_columns": wl.n_cols},
+          precisions=per,
+          gradient_sample={"regions": nsub, "codon_columns": sub_cols,
+                           "mle_grad_scored_columns_per_s": sub_cols / dt_mle,
+                           "mean_gradient_evaluations": float(g["n_evals"].mean()),
+                           "per_branch_gradient_columns_per_s": sub_cols / dt_b,
+                           "branches": int(gbr.shape[-1])})
 
 
 if __name__ == "__main__":

@@ -120,6 +120,20 @@ int pcsf_model_transition(pcsf_model *m, int model, double rho, double *P_out);
  * are [off[r], off[r+1]). */
 int pcsf_batch_create(pcsf_model *m, const uint8_t *codes, int64_t n_cols,
                       const int64_t *region_offsets, int32_t n_regions, pcsf_batch **out);
+/* A batch of every reading frame of packed nucleotide regions: PhyloCSF's
+ * AsIs candidate regions with --frames 1, 3 or 6 (src/PhyloCSF.ml:198-206)
+ * read by pleaves (:219-246), built on the GPU.  nuc: n_leaves rows of
+ * n_nuc characters in tree-leaf order (row l at nuc + l * n_nuc; a species
+ * absent from the alignment is a row of '-'); region r spans characters
+ * [region_offsets[r], region_offsets[r+1]) of every row.  Batch region
+ * r * frames + f is frame f of region r: forward strand offsets 0, 1, 2,
+ * then reverse-complement offsets 0, 1, 2, each floor((len - f%3) / 3)
+ * codon columns.  Codons with a character outside ACGTacgt are
+ * `Marginalize (the reference's rejection of other characters, in
+ * Code.DNA.revcomp, is the caller's check). */
+int pcsf_batch_create_frames(pcsf_model *m, const uint8_t *nuc, int64_t n_nuc,
+                             const int64_t *region_offsets, int32_t n_regions, int frames,
+                             pcsf_batch **out);
 void pcsf_batch_destroy(pcsf_batch *b);
 int64_t pcsf_batch_columns(const pcsf_batch *b);
 
@@ -218,6 +232,15 @@ int pcsf_simulate(pcsf_model *m, int32_t n_regions, const int64_t *region_offset
                   const int32_t *region_model, const double *region_rho, int gap_mode,
                   double gap_p, double gap_run, double n_p, uint64_t seed, int n_threads,
                   uint8_t *codes_out);
+/* The same alignments as nucleotide characters: nuc_out holds n_leaves rows
+ * of 3 * n_cols characters (row l at nuc_out + l * 3 * n_cols; codon column
+ * c at characters 3c .. 3c+2); a gap is "---", an N codon carries 'N' at
+ * one of its three positions.  pcsf_batch_create_frames of this output
+ * with frames = 1 gives the codes of pcsf_simulate. */
+int pcsf_simulate_nucleotides(pcsf_model *m, int32_t n_regions, const int64_t *region_offsets,
+                              const int32_t *region_model, const double *region_rho,
+                              int gap_mode, double gap_p, double gap_run, double n_p,
+                              uint64_t seed, int n_threads, uint8_t *nuc_out);
 
 #ifdef __cplusplus
 }

@@ -30,7 +30,7 @@ EXPORTED = [
     "pcsf_batch_create", "pcsf_batch_destroy", "pcsf_batch_columns", "pcsf_forward",
     "pcsf_backward", "pcsf_batch_device_results", "pcsf_batch_last_timing",
     "pcsf_lik_columns", "pcsf_simulate", "pcsf_autotune", "pcsf_set_variant", "pcsf_score",
-    "pcsf_model_specialized",
+    "pcsf_model_specialized", "pcsf_batch_create_frames", "pcsf_simulate_nucleotides",
     "pcsf_program",
 ]
 PCSF_STRATEGY_FIXED = 0
@@ -98,6 +98,8 @@ def lib():
     L.pcsf_model_info.argtypes = [vp, ip, ip, ip, ip]
     L.pcsf_model_transition.argtypes = [vp, C.c_int, C.c_double, dp]
     L.pcsf_batch_create.argtypes = [vp, u8p, C.c_int64, i64p, C.c_int32, C.POINTER(vp)]
+    L.pcsf_batch_create_frames.argtypes = [vp, u8p, C.c_int64, i64p, C.c_int32, C.c_int,
+                                           C.POINTER(vp)]
     L.pcsf_batch_destroy.argtypes = [vp]
     L.pcsf_batch_destroy.restype = None
     L.pcsf_batch_columns.argtypes = [vp]
@@ -109,6 +111,7 @@ def lib():
     L.pcsf_lik_columns.argtypes = [vp, C.c_int, C.c_int, i32p, dp, dp, i32p, C.c_int64, dp, dp]
     L.pcsf_simulate.argtypes = [vp, C.c_int32, i64p, i32p, dp, C.c_int, C.c_double,
                                 C.c_double, C.c_double, C.c_uint64, C.c_int, u8p]
+    L.pcsf_simulate_nucleotides.argtypes = L.pcsf_simulate.argtypes
     L.pcsf_autotune.argtypes = [vp, C.c_int, C.c_int, dp, ip]
     L.pcsf_set_variant.argtypes = [vp, C.c_int, C.c_int]
     L.pcsf_model_specialized.argtypes = [vp, C.c_int, ip]

@@ -223,6 +223,24 @@ class PhyloCSFModel:
             raise_for(rc, "pcsf_simulate failed")
         return codes, off
 
+    def simulate_nucleotides(self, lengths, region_model, region_rho, gap_mode=0, gap_p=0.0,
+                             gap_run=5.0, n_p=0.0, seed=1, n_threads=0):
+        """The alignments of simulate() as nucleotide characters:
+        -> (nuc [n_leaves, 3 * n_cols] uint8 in tree-leaf order, nucleotide
+        region offsets [R+1])."""
+        lengths = np.asarray(lengths, dtype=np.int64)
+        off = np.zeros(len(lengths) + 1, dtype=np.int64)
+        np.cumsum(lengths, out=off[1:])
+        nuc = np.zeros((self.tree.nl, 3 * int(off[-1])), dtype=np.uint8)
+        rm = np.ascontiguousarray(region_model, dtype=np.int32)
+        rr = np.ascontiguousarray(region_rho, dtype=np.float64)
+        rc = lib().pcsf_simulate_nucleotides(self.h, len(lengths), _i64(off), _i32(rm), _dp(rr),
+                                             gap_mode, gap_p, gap_run, n_p, seed, n_threads,
+                                             _u8(nuc))
+        if rc:
+            raise_for(rc, "pcsf_simulate_nucleotides failed")
+        return nuc, 3 * off
+
     def close(self):
         if getattr(self, "h", None):
             lib().pcsf_model_destroy(self.h)
@@ -250,6 +268,26 @@ class Batch:
         model.ctx.check(lib().pcsf_batch_create(model.h, _u8(codes), self.n_cols, _i64(offsets),
                                                 self.n_regions, C.byref(h)))
         self.h = h
+
+    @classmethod
+    def frames(cls, model: PhyloCSFModel, nuc: np.ndarray, offsets: np.ndarray, frames: int):
+        """Every reading frame of packed nucleotide regions, built on the GPU
+        (pcsf_batch_create_frames): nuc is [n_leaves, n_nuc] characters in
+        tree-leaf order; batch region r * frames + f is frame f of region r."""
+        nuc = np.ascontiguousarray(nuc, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        if nuc.ndim != 2 or nuc.shape[0] != model.tree.nl:
+            raise InvalidArgument(-1, "nuc must be [n_leaves, n_nuc]")
+        b = cls.__new__(cls)
+        b.model = model
+        h = C.c_void_p()
+        model.ctx.check(lib().pcsf_batch_create_frames(model.h, _u8(nuc), nuc.shape[1],
+                                                       _i64(offsets), len(offsets) - 1,
+                                                       int(frames), C.byref(h)))
+        b.h = h
+        b.n_regions = (len(offsets) - 1) * int(frames)
+        b.n_cols = int(lib().pcsf_batch_columns(h))
+        return b
 
     @classmethod
     def from_regions(cls, model, regions):

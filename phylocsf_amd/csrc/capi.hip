@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -48,9 +49,9 @@ hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0) {
     if (e == hipSuccess) e = x;
   };
   const size_t n = (size_t)cap * nbr;
-  chk(dalloc(&ps.d_PT32, n * 4096));
+  chk(dalloc(&ps.d_PT32, n * kLeafRows * 64));
   chk(dalloc(&ps.d_Pfrag, n * kTileFloats));
-  chk(dalloc(&ps.d_lnPcc, n * 64));
+  chk(dalloc(&ps.d_lnPcc, n * kLeafRows));
   chk(dalloc(&ps.d_rho, (size_t)cap));
   chk(dalloc(&ps.d_active, (size_t)cap));
   if (fp64) chk(dalloc(&ps.d_P64T, n * 4096));
@@ -184,7 +185,8 @@ int ensure_aligned(pcsf_batch* b) {
 
 // Bytes of one transition set on the device.
 size_t set_bytes(const pcsf_model* m, bool fp64) {
-  return (size_t)m->nbr * (4096 * 4 + kTileFloats * 4 + 64 * 8 + (fp64 ? 4096 * 8 : 0));
+  return (size_t)m->nbr *
+         (kLeafRows * 64 * 4 + kTileFloats * 4 + kLeafRows * 8 + (fp64 ? 4096 * 8 : 0));
 }
 
 // Forward with one tree scale per (region, model): rho[r*nm+q].  active
@@ -631,6 +633,70 @@ int pcsf_batch_create(pcsf_model* m, const uint8_t* codes, int64_t ncols,
   return PCSF_OK;
 }
 
+int pcsf_batch_create_frames(pcsf_model* m, const uint8_t* nuc, int64_t n_nuc,
+                             const int64_t* roff, int32_t nreg, int frames, pcsf_batch** out) {
+  if (!m || !out) return PCSF_EINVAL;
+  pcsf_ctx* c = m->ctx;
+  *out = nullptr;
+  if (n_nuc < 0 || nreg < 0 || (n_nuc > 0 && !nuc) || !roff ||
+      !(frames == 1 || frames == 3 || frames == 6))
+    return set_err(c, PCSF_EINVAL, "pcsf_batch_create_frames: bad arguments");
+  if (roff[0] != 0 || roff[nreg] != n_nuc)
+    return set_err(c, PCSF_EINVAL,
+                   "pcsf_batch_create_frames: region offsets must span [0, n_nuc]");
+  std::vector<int64_t> foff((size_t)nreg * frames + 1, 0);
+  for (int32_t r = 0; r < nreg; ++r) {
+    const int64_t len = roff[r + 1] - roff[r];
+    if (len < 0)
+      return set_err(c, PCSF_EINVAL,
+                     "pcsf_batch_create_frames: region offsets must not decrease");
+    for (int f = 0; f < frames; ++f) {
+      const int64_t k = (size_t)r * frames + f, rest = len - f % 3;
+      foff[k + 1] = foff[k] + (rest >= 3 ? rest / 3 : 0);
+    }
+  }
+  const int64_t ncols = foff.back();
+  const int32_t nbr = (int32_t)(foff.size() - 1);
+  auto b = new (std::nothrow) pcsf_batch();
+  if (!b) return PCSF_ENOMEM;
+  std::unique_ptr<pcsf_batch, void (*)(pcsf_batch*)> guard(b, pcsf_batch_destroy);
+  b->m = m;
+  b->ncols = ncols;
+  b->nreg = nbr;
+  b->offsets = foff;
+  b->ld = std::max<int64_t>(kGroupCols, (ncols + kGroupCols - 1) / kGroupCols * kGroupCols);
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, dalloc(&b->d_codes, (size_t)m->nl * b->ld));
+  HIP_TRY(c, dalloc(&b->d_offsets, (size_t)nbr + 1));
+  HIP_TRY(c, dalloc(&b->d_logz, (size_t)m->nm * b->ld));
+  HIP_TRY(c, dalloc(&b->d_colelpr, (size_t)m->nm * b->ld));
+  HIP_TRY(c, dalloc(&b->d_loglik, std::max<size_t>(1, (size_t)nbr * m->nm)));
+  HIP_TRY(c, dalloc(&b->d_elpr, std::max<size_t>(1, (size_t)nbr * m->nm)));
+  HIP_TRY(c, hipMemcpy(b->d_offsets, foff.data(), sizeof(int64_t) * foff.size(),
+                       hipMemcpyHostToDevice));
+  uint8_t* d_nuc = nullptr;
+  int64_t* d_roff = nullptr;
+  const size_t bytes = (size_t)n_nuc * m->nl;
+  hipError_t e = dalloc(&d_nuc, std::max<size_t>(bytes, 1));
+  if (e == hipSuccess) e = dalloc(&d_roff, (size_t)nreg + 1);
+  if (e == hipSuccess && bytes) e = hipMemcpy(d_nuc, nuc, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(d_roff, roff, sizeof(int64_t) * (nreg + 1), hipMemcpyHostToDevice);
+  // padding columns stay `Marginalize
+  if (e == hipSuccess)
+    e = hipMemsetAsync(b->d_codes, PCSF_MARGINALIZE, (size_t)m->nl * b->ld, c->stream);
+  if (e == hipSuccess)
+    e = launch_frames(d_nuc, n_nuc, d_roff, b->d_offsets, nreg, frames, m->nl, b->d_codes, b->ld,
+                      c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dfree(d_nuc);
+  dfree(d_roff);
+  HIP_TRY(c, e);
+  for (auto& ev : b->ev) HIP_TRY(c, hipEventCreate(&ev));
+  *out = guard.release();
+  return PCSF_OK;
+}
+
 void pcsf_batch_destroy(pcsf_batch* b) {
   if (!b) return;
   dfree(b->d_codes);
@@ -893,6 +959,12 @@ int score_mle(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   }
   std::vector<double> rho(np, 1.0), L(np), E(np);
   std::vector<uint8_t> act(np);
+  // PCSF_TRACE_MLE=<pair index r*nm+q>: every evaluation of that pair to stderr
+  const char* tr = std::getenv("PCSF_TRACE_MLE");
+  const long trace = tr ? std::strtol(tr, nullptr, 10) : -1;
+  if (trace >= 0 && (size_t)trace < np)
+    std::fprintf(stderr, "mle pair %ld: lo %.17g hi %.17g init %.17g v %.17g (%.17g) phase %d\n",
+                 trace, Llo[trace], Lhi[trace], L1[trace], v, Lv[trace], (int)st[trace].phase);
   for (int round = 0; round < 100000; ++round) {
     bool any = false;
     for (size_t k = 0; k < np; ++k) {
@@ -910,6 +982,9 @@ int score_mle(pcsf_batch* b, int precision, std::vector<PairState>& st) {
       if (!act[k]) continue;
       PairState& s = st[k];
       const double x = s.pending, l = L[k], e = E[k];
+      if ((long)k == trace)
+        std::fprintf(stderr, "mle pair %ld round %d phase %d: x %.17g lpr %.17g\n", trace,
+                     round, (int)s.phase, x, l);
       s.evals++;
       s.remember(x, l, e);
       switch (s.phase) {

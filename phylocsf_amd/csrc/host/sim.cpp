@@ -86,11 +86,15 @@ int draw(const double* cum, int k, double u) {
 
 }  // namespace
 
-extern "C" int pcsf_simulate(pcsf_model* m, int32_t nreg, const int64_t* off,
-                             const int32_t* region_model, const double* region_rho,
-                             int gap_mode, double gap_p, double gap_run, double n_p,
-                             uint64_t seed, int n_threads, uint8_t* codes) {
-  if (!m || nreg < 0 || !off || !region_model || !region_rho || !codes) return PCSF_EINVAL;
+// Writes codes (n_cols x nl) or, if nuc is given instead, nucleotide
+// characters: leaf j's row at nuc + j * 3 * n_cols, codon column c at
+// characters 3c..3c+2; a gap is "---" and an N codon has 'N' at one of its
+// three positions (chosen from the same uniform that decided the N).
+static int simulate(pcsf_model* m, int32_t nreg, const int64_t* off, const int32_t* region_model,
+             const double* region_rho, int gap_mode, double gap_p, double gap_run, double n_p,
+             uint64_t seed, int n_threads, uint8_t* codes, uint8_t* nuc) {
+  if (!m || nreg < 0 || !off || !region_model || !region_rho || (!codes && !nuc))
+    return PCSF_EINVAL;
   if (gap_p < 0 || gap_p >= 1 || n_p < 0 || n_p > 1 || (gap_mode == 1 && !(gap_run >= 1)))
     return PCSF_EINVAL;
   for (int32_t r = 0; r < nreg; ++r)
@@ -148,8 +152,19 @@ extern "C" int pcsf_simulate(pcsf_model* m, int32_t nreg, const int64_t* off,
           for (int i = n - 2; i >= 0; --i)
             state[i] = draw(cum.data() + (size_t)i * 4096 + state[m->parent[i]] * 64, 64,
                             st.uniform());
-          uint8_t* row = codes + (size_t)c * nl;
-          row[0] = (uint8_t)state[0];
+          static const char kNuc[4] = {'A', 'C', 'G', 'T'};
+          const int64_t ntot = off[nreg];
+          auto put = [&](int j, int code, bool gap, int npos) {
+            if (codes) {
+              codes[(size_t)c * nl + j] = (gap || npos >= 0) ? (uint8_t)PCSF_MARGINALIZE
+                                                             : (uint8_t)code;
+              return;
+            }
+            uint8_t* w = nuc + (size_t)j * 3 * ntot + 3 * (size_t)c;
+            for (int k = 0; k < 3; ++k)
+              w[k] = gap ? '-' : (k == npos) ? 'N' : kNuc[(code >> (4 - 2 * k)) & 3];
+          };
+          put(0, state[0], false, -1);
           for (int j = 1; j < nl; ++j) {
             bool gap;
             const double u = st.uniform();
@@ -160,8 +175,9 @@ extern "C" int pcsf_simulate(pcsf_model* m, int32_t nreg, const int64_t* off,
             } else {
               gap = u < gap_p;
             }
-            const bool nn = st.uniform() < n_p;
-            row[j] = (gap || nn) ? (uint8_t)PCSF_MARGINALIZE : (uint8_t)state[j];
+            const double un = st.uniform();
+            const int npos = (un < n_p) ? std::min(2, (int)(3.0 * un / n_p)) : -1;
+            put(j, state[j], gap, gap ? -1 : npos);
           }
         }
       }
@@ -171,4 +187,23 @@ extern "C" int pcsf_simulate(pcsf_model* m, int32_t nreg, const int64_t* off,
   for (int f : fails)
     if (f) status = PCSF_EFAIL;
   return status;
+}
+
+extern "C" int pcsf_simulate(pcsf_model* m, int32_t nreg, const int64_t* off,
+                             const int32_t* region_model, const double* region_rho,
+                             int gap_mode, double gap_p, double gap_run, double n_p,
+                             uint64_t seed, int n_threads, uint8_t* codes) {
+  if (!codes) return PCSF_EINVAL;
+  return simulate(m, nreg, off, region_model, region_rho, gap_mode, gap_p, gap_run, n_p, seed,
+                  n_threads, codes, nullptr);
+}
+
+extern "C" int pcsf_simulate_nucleotides(pcsf_model* m, int32_t nreg, const int64_t* off,
+                                         const int32_t* region_model, const double* region_rho,
+                                         int gap_mode, double gap_p, double gap_run,
+                                         double n_p, uint64_t seed, int n_threads,
+                                         uint8_t* nuc) {
+  if (!nuc) return PCSF_EINVAL;
+  return simulate(m, nreg, off, region_model, region_rho, gap_mode, gap_p, gap_run, n_p, seed,
+                  n_threads, nullptr, nuc);
 }

@@ -13,14 +13,16 @@
 constexpr int kTileCols = 16;   // columns per wave
 constexpr int kGroupCols = 128; // columns per workgroup (8 waves)
 constexpr int kTileFloats = 4096 + 64;  // P tile staged in LDS: fragments + diagonal
+constexpr int kLeafRows = 65;  // leaf messages per branch: codons 0..63, 64 = `Marginalize
 
 // Transition-matrix sets on the device: set s is model s % nm at tree scale
 // rho[s] (the fixed strategy uses nm sets; per-region scales use one set per
 // (region, model) of a chunk of regions).
 // Formats of the P tile (PSets::d_Pfrag, kTileFloats floats per branch):
 //   kFragF32   off-diagonal FP32 in v_mfma_f32_16x16x4f32 A order + diagonal
-//   kFragF16   off-diagonal x 2^14 as FP16 hi, then FP16 lo, in
-//              v_mfma_f32_16x16x32_f16 A order (4096 halves each) + diagonal
+//   kFragF16   off-diagonal x 2^14 as FP16 hi, then the FP16 residual x 2^11
+//              (lo), in v_mfma_f32_16x16x32_f16 A order (4096 halves each)
+//              + diagonal
 //   kFragBF16  the same in BF16, unscaled
 // The diagonal part is always 64 FP32 values P[a][a] - 1 at float 4096.
 constexpr int kFragF32 = 0, kFragF16 = 1, kFragBF16 = 2;
@@ -31,10 +33,11 @@ struct PSets {
   int fmt = -1;                // format of d_Pfrag as last built
   bool fp64 = false;           // P64T allocated
   double* d_P64T = nullptr;    // cap * nbr * 4096 FP64 P^T (P64T[b][a] = P[a][b])
-  float* d_PT32 = nullptr;     // cap * nbr * 4096 FP32 P^T[c][a] = P[a][c] / P[c][c]
+  float* d_PT32 = nullptr;     // cap * nbr * 65 * 64 FP32 leaf messages: row c < 64 is
+                               //   P[a][c] / P[c][c], row 64 all ones (`Marginalize)
   float* d_Pfrag = nullptr;    // cap * nbr * kTileFloats: off-diagonal of P in MFMA
                                //   A-fragment order, then the 64 values P[a][a] - 1
-  double* d_lnPcc = nullptr;   // cap * nbr * 64 FP64 ln P[c][c]
+  double* d_lnPcc = nullptr;   // cap * nbr * 65 FP64 ln P[c][c] (0 for row 64)
   double* d_rho = nullptr;     // cap tree scales
   uint8_t* d_active = nullptr; // cap flags: 0 = set not needed by this launch
   // cherry tables (fixed scales only): message P_c (P_a[:,x] o P_b[:,y]) of
@@ -145,6 +148,9 @@ hipError_t launch_prune_fp64_regions(const pcsf_model* m, const pcsf_batch* b,
 hipError_t launch_prune_mfma_regions(const pcsf_model* m, const pcsf_batch* b, int precision,
                                      int variant, const RegionChunk& ch, hipStream_t s);
 hipError_t launch_align_codes(const pcsf_batch* b, hipStream_t s);
+hipError_t launch_frames(const uint8_t* nuc, int64_t nlen, const int64_t* d_roff,
+                         const int64_t* d_foff, int32_t nreg, int frames, int nl, uint8_t* dst,
+                         int64_t ld, hipStream_t s);
 hipError_t launch_adjoint_fp64(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
                                double* grad_branch, hipStream_t s);
 hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& ch,

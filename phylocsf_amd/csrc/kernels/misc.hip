@@ -142,6 +142,74 @@ hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uin
   return hipGetLastError();
 }
 
+namespace {
+
+__device__ __forceinline__ int nuc_index(uint8_t ch) {  // Code.DNA.index; -1: not ACGTacgt
+  switch (ch) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return -1;
+  }
+}
+
+// Codon codes of every reading frame of packed nucleotide regions: the
+// candidate regions of PhyloCSF's AsIs mode with --frames 3 or 6
+// (src/PhyloCSF.ml:198-206) read by pleaves (:219-246).  One workgroup per
+// (region, frame); frame f reads codons from offset f % 3 of the forward
+// strand (f < 3) or of the reverse complement (Code.DNA.revcomp: the
+// complement of A,C,G,T is T,G,C,A, i.e. index 3 - i).  A codon with any
+// character outside ACGTacgt is `Marginalize.  nuc holds nl rows of nlen
+// characters; dst is the batch's leaf-major code array.
+__global__ __launch_bounds__(256) void frames_kernel(const uint8_t* __restrict__ nuc,
+                                                     int64_t nlen,
+                                                     const int64_t* __restrict__ roff,
+                                                     const int64_t* __restrict__ foff, int F,
+                                                     int nl, uint8_t* __restrict__ dst,
+                                                     int64_t ld) {
+  const int64_t rf = blockIdx.x;
+  const int64_t r = rf / F;
+  const int f = (int)(rf - r * F), o = f % 3;
+  const bool rc = f >= 3;
+  const int64_t base = roff[r], L = roff[r + 1] - base;
+  const int64_t col0 = foff[rf], count = foff[rf + 1] - col0;
+  for (int64_t e = threadIdx.x; e < count * nl; e += blockDim.x) {
+    const int leaf = (int)(e / count);
+    const int64_t j = e - (int64_t)leaf * count;
+    const uint8_t* row = nuc + (size_t)leaf * nlen + base;
+    int a, b, c;
+    if (!rc) {
+      const int64_t p = o + 3 * j;
+      a = nuc_index(row[p]);
+      b = nuc_index(row[p + 1]);
+      c = nuc_index(row[p + 2]);
+    } else {
+      const int64_t q = L - 1 - (o + 3 * j);
+      a = nuc_index(row[q]);
+      b = nuc_index(row[q - 1]);
+      c = nuc_index(row[q - 2]);
+      a = a < 0 ? -1 : 3 - a;
+      b = b < 0 ? -1 : 3 - b;
+      c = c < 0 ? -1 : 3 - c;
+    }
+    dst[(size_t)leaf * ld + col0 + j] =
+        (a < 0 || b < 0 || c < 0) ? (uint8_t)PCSF_MARGINALIZE : (uint8_t)(16 * a + 4 * b + c);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_frames(const uint8_t* nuc, int64_t nlen, const int64_t* d_roff,
+                         const int64_t* d_foff, int32_t nreg, int frames, int nl, uint8_t* dst,
+                         int64_t ld, hipStream_t s) {
+  const int64_t blocks = (int64_t)nreg * frames;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(frames_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nuc, nlen, d_roff,
+                     d_foff, frames, nl, dst, ld);
+  return hipGetLastError();
+}
+
 hipError_t launch_region_reduce(const pcsf_batch* b, hipStream_t s) {
   if (b->nreg == 0) return hipSuccess;
   dim3 grid(b->nreg, b->m->nm);

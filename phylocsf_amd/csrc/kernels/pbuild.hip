@@ -10,9 +10,10 @@
 // Outputs (all [model][branch][4096]):
 //   P64T  FP64 P transposed (P64T[b][a] = P[a][b]): coalesced GEMV operand of
 //         the FP64 kernel and its leaf gathers.
-//   PT32  FP32 leaf messages of the MFMA kernel: PT32[c][a] = P[a][c] / P[c][c],
-//         with lnPcc[c] = ln P[c][c] in FP64 (the dominant entry of a leaf
-//         message is exactly 1 and its value travels in FP64).
+//   PT32  FP32 leaf messages of the MFMA kernel, 65 rows of 64: row c < 64 is
+//         P[a][c] / P[c][c] with lnPcc[c] = ln P[c][c] in FP64 (the dominant
+//         entry of a leaf message is exactly 1 and its value travels in FP64);
+//         row 64 is the all-ones message of `Marginalize (lnPcc 0).
 //   Pfrag tile of kTileFloats floats in format fmt: the off-diagonal of P
 //         in the A-operand order of the MFMA the kernel uses (diagonal
 //         entries zero) -- FP32 for v_mfma_f32_16x16x4f32, or hi and lo
@@ -93,13 +94,18 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
   }
   __syncthreads();
   const size_t base = ((size_t)set * nbr + br) * 4096;
+  const size_t lbase = ((size_t)set * nbr + br) * (kLeafRows * 64);
   for (int e = tid; e < 4096; e += 256) {
     const int b = e >> 6, a = e & 63;  // transposed: [b][a] = P[a][b]
     const double p = Vs[a * 64 + b];
     if (P64T) P64T[base + e] = p;
-    PT32[base + e] = (float)(p / Vs[b * 64 + b]);
+    PT32[lbase + e] = (float)(p / Vs[b * 64 + b]);
   }
-  if (tid < 64) lnPcc[((size_t)set * nbr + br) * 64 + tid] = log(Vs[tid * 64 + tid]);
+  if (tid < 64) {
+    PT32[lbase + 4096 + tid] = 1.0f;
+    lnPcc[((size_t)set * nbr + br) * kLeafRows + tid] = log(Vs[tid * 64 + tid]);
+  }
+  if (tid == 64) lnPcc[((size_t)set * nbr + br) * kLeafRows + 64] = 0.0;
   const size_t fbase = ((size_t)set * nbr + br) * kTileFloats;
   if (fmt == kFragF32) {
     // fragment order of v_mfma_f32_16x16x4f32's A operand with the K index
@@ -116,7 +122,9 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
     // v_mfma_f32_16x16x32_{f16,bf16} A order, K permuted the same way:
     //   e = ((mo*2 + st)*64 + lane)*8 + j
     //   row = 16*mo + lane%16, col = 16*(2*st + j/4) + 4*(lane/16) + j%4
-    // hi = x rounded to nearest, lo = the residual x - hi rounded to nearest
+    // hi = x rounded to nearest, lo = the residual x - hi rounded to nearest;
+    // FP16: x = P * 2^14 and lo carries a further 2^11 (a normal number
+    // wherever hi is: the matrix cores read FP16 subnormals as zero)
     const double sc = (fmt == kFragF16) ? 16384.0 : 1.0;
     for (int e = tid; e < 4096; e += 256) {
       const int j = e & 7, lane = (e >> 3) & 63, st = (e >> 9) & 1, mo = e >> 10;
@@ -127,7 +135,7 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
         _Float16* h = reinterpret_cast<_Float16*>(Pfrag + fbase);
         const _Float16 hi = (_Float16)(float)x;
         h[e] = hi;
-        h[4096 + e] = (_Float16)(float)(x - (double)(float)hi);
+        h[4096 + e] = (_Float16)(float)((x - (double)(float)hi) * 2048.0);
       } else {
         __bf16* h = reinterpret_cast<__bf16*>(Pfrag + fbase);
         const __bf16 hi = (__bf16)(float)x;

@@ -42,12 +42,16 @@
 //  * GEMM precision (template PREC): FP32 on v_mfma_f32_16x16x4f32; or P and
 //    the partial enter v_mfma_f32_16x16x32_{f16,bf16} in half precision with
 //    FP32 accumulation.  FP16 operands are scaled by 2^14 (P's off-diagonal
-//    entries are below 1 and partials are normalised to a maximum below 1, so
-//    nothing overflows and values down to 2^-28 stay normal) and the product
-//    is scaled back by 2^-28, exactly.  The split-FP16 precision carries each
-//    operand as hi + lo (lo = the FP16 rounding of the residual) and forms
-//    hi*hi + hi*lo + lo*hi: about 22 significant bits per product at three
-//    FP16 MFMAs, against sixteen FP32 MFMAs for the same 64x64x16 product.
+//    entries are below 1; the live tiles of the FP16 kernels are kept with
+//    their column maximum in [2^13, 2^14) by the rescaling itself, so nothing
+//    overflows and values down to 2^-28 of the maximum stay normal) and P's
+//    2^14 is taken off the product exactly.  The split-FP16 precision carries each
+//    operand as hi + lo, lo = the FP16 rounding of the residual scaled by 2^11
+//    (so that it stays a normal number: the matrix cores treat FP16
+//    subnormals as zero, which measurably biased an unscaled lo), and forms
+//    hi*hi in one accumulator and hi*lo + lo*hi in another: about 22
+//    significant bits per product at three FP16 MFMAs, against sixteen FP32
+//    MFMAs for the same 64x64x16 product.
 //    Live partials stay FP32 in registers in every precision; the diagonal
 //    of P stays in the FP32 VALU term.
 #include <type_traits>
@@ -63,8 +67,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // precisions of the GEMM (values of PCSF_FP32 .. PCSF_BF16)
 constexpr int kPrecF32 = PCSF_FP32, kPrecF16S = PCSF_FP16S, kPrecF16 = PCSF_FP16,
               kPrecBF16 = PCSF_BF16;
-constexpr float kF16Scale = 16384.0f;                    // 2^14
-constexpr float kF16Unscale = 1.0f / (16384.0f * 16384.0f);  // 2^-28
+constexpr int kF16Exp = 14;                 // FP16 operands are scaled by 2^14
+constexpr float kF16Unscale = 1.0f / 16384.0f;  // P's scale, taken off the product
+constexpr float kLoScale = 2048.0f;             // lo parts of split FP16 operands: x 2^11
+constexpr float kLoUnscale = 1.0f / 2048.0f;
 
 struct MfmaParams {
   const uint8_t* codes;
@@ -142,27 +148,29 @@ struct Gathered {
   bool marg;
 };
 
-// Loads the normalised column of P of leaf `leaf` for this lane's codon.
+// Loads the leaf message of leaf `leaf` for this lane's codon: the
+// normalised column of P, or row 64 (all ones) for `Marginalize.
 __device__ __forceinline__ void gather(Gathered& g, const MfmaParams& p, const uint8_t* codes_s,
                                        int gc, int set, int leaf, int ccol, int grp) {
   const int code = codes_s[leaf * gc + ccol];
   g.marg = code == PCSF_MARGINALIZE;
-  const int c = g.marg ? 0 : code;
-  const size_t bc = ((size_t)set * p.nbr + leaf) * 64 + c;
+  const int c = g.marg ? 64 : code;
+  const size_t bc = ((size_t)set * p.nbr + leaf) * kLeafRows + c;
   const float* src = p.PT32 + bc * 64 + 4 * grp;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) g.t.r[mb] = *reinterpret_cast<const f32x4*>(src + 16 * mb);
   g.ln = p.lnPcc[bc];
 }
 
-// Applies a gathered leaf: `Marginalize is the all-ones message.
 __device__ __forceinline__ const Tile& leaf_tile(Gathered& g, double& ls) {
-#pragma unroll
-  for (int mb = 0; mb < 4; ++mb) g.t.r[mb] = g.marg ? (f32x4)(1.0f) : g.t.r[mb];
-  ls += g.marg ? 0.0 : g.ln;
+  ls += g.ln;
   return g.t;
 }
 
+// s *= o, then every column is scaled by the power of two that brings its
+// maximum to [2^(KE-1), 2^KE); E accumulates the exponents, so the true
+// product of the column is s * 2^E.
+template <int KE>
 __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
   float mx = 0.0f;
 #pragma unroll
@@ -177,8 +185,8 @@ __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) s.r[mb][v] = __builtin_amdgcn_ldexpf(s.r[mb][v], -e);
-  E += e;
+    for (int v = 0; v < 4; ++v) s.r[mb][v] = __builtin_amdgcn_ldexpf(s.r[mb][v], KE - e);
+  E += e - KE;
 }
 
 // 64x64 (off-diagonal P, LDS tile Ab) times the 64x16 tile s in half
@@ -189,21 +197,26 @@ __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
 // its column: exactly what the accumulator layout leaves in registers
 // s.r[2*st + j/4][j%4].
 template <int PREC>
-__device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], const Tile& s, const float* Ab,
-                                          int lane) {
+__device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], f32x4 (&accx)[4], const Tile& s,
+                                          const float* Ab, int lane) {
   constexpr bool kBF = PREC == kPrecBF16;
   using V8 = typename std::conditional<kBF, bf16x8, f16x8>::type;
   using E = typename std::conditional<kBF, __bf16, _Float16>::type;
-  const float sc = kBF ? 1.0f : kF16Scale;
   V8 bh[2], bl[2];
 #pragma unroll
   for (int st = 0; st < 2; ++st)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const float x = s.r[2 * st + (j >> 2)][j & 3] * sc;
-      const E h = (E)x;
-      bh[st][j] = h;
-      if constexpr (PREC == kPrecF16S) bl[st][j] = (E)(x - (float)h);
+      const float x = s.r[2 * st + (j >> 2)][j & 3];
+      if constexpr (PREC == kPrecF16S) {
+        // hi = x cut to 11 significant bits (exact in FP16), lo = the exact
+        // remainder, scaled and rounded
+        const float h = __uint_as_float(__float_as_uint(x) & 0xFFFFE000u);
+        bh[st][j] = (E)h;
+        bl[st][j] = (E)((x - h) * kLoScale);
+      } else {
+        bh[st][j] = (E)x;
+      }
     }
   const V8* Ah = reinterpret_cast<const V8*>(Ab);
   const V8* Al = Ah + 512;
@@ -213,24 +226,22 @@ __device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], const Tile& s, const 
     if constexpr (kBF) return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     else return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   };
-  if constexpr (PREC == kPrecF16S) {  // small cross terms first
+  if constexpr (PREC == kPrecF16S) {  // cross terms, carrying the lo scale
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo) accx[mo] = (f32x4)(0.0f);
 #pragma unroll
     for (int st = 0; st < 2; ++st)
 #pragma unroll
       for (int mo = 0; mo < 4; ++mo) {
         const int f = (mo * 2 + st) * 64 + lane;
-        acc[mo] = mma(Al[f], bh[st], acc[mo]);
-        acc[mo] = mma(Ah[f], bl[st], acc[mo]);
+        accx[mo] = mma(Al[f], bh[st], accx[mo]);
+        accx[mo] = mma(Ah[f], bl[st], accx[mo]);
       }
   }
 #pragma unroll
   for (int st = 0; st < 2; ++st)
 #pragma unroll
     for (int mo = 0; mo < 4; ++mo) acc[mo] = mma(Ah[(mo * 2 + st) * 64 + lane], bh[st], acc[mo]);
-  if constexpr (!kBF) {
-#pragma unroll
-    for (int mo = 0; mo < 4; ++mo) acc[mo] *= kF16Unscale;
-  }
 }
 
 // One wave's walk of the op program over its 16 columns, and the ops.  The
@@ -239,6 +250,9 @@ __device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], const Tile& s, const 
 // the arithmetic is the same code in both.
 template <int PREC, int WPG, bool PREF>
 struct Walk {
+  // live tiles of the FP16 GEMMs are kept scaled by 2^14 (their maximum in
+  // [2^13, 2^14)), the scale their GEMM operand needs
+  static constexpr int KE = (PREC == kPrecF16S || PREC == kPrecF16) ? kF16Exp : 0;
   static constexpr int NT = 64 * WPG;        // threads
   static constexpr int PER = 4096 / NT / 4;  // float4 of the fragments per thread
   static constexpr int GC = 16 * WPG;        // columns per workgroup
@@ -281,7 +295,7 @@ struct Walk {
     Gathered& g0 = first_leaf(a0, use_pf);
     s = leaf_tile(g0, LS);
     k = __ballot(g0.marg && g1.marg);
-    mul_rescale(s, leaf_tile(g1, LS), E);
+    mul_rescale<KE>(s, leaf_tile(g1, LS), E);
   }
   // s = message above the cherry (a0, a1), from table `tab`
   __device__ __forceinline__ void cherryt(Tile& s, uint64_t& k, int a0, int a1, int tab) {
@@ -292,6 +306,8 @@ struct Walk {
     const float* src = p.ctab + e * 64 + 4 * grp;
 #pragma unroll
     for (int mbk = 0; mbk < 4; ++mbk) s.r[mbk] = *reinterpret_cast<const f32x4*>(src + 16 * mbk);
+    // (a table message always meets a sibling in LEAFMUL / MULPOP, whose
+    // rescaling sets the tile scale, before it reaches a GEMM)
     LS += p.clnt[e];
     k = __ballot(ma && mb);
   }
@@ -299,12 +315,12 @@ struct Walk {
   __device__ __forceinline__ void leafmul(Tile& s, uint64_t& k, int a0, bool use_pf) {
     Gathered& g0 = first_leaf(a0, use_pf);
     k &= __ballot(g0.marg);
-    mul_rescale(s, leaf_tile(g0, LS), E);
+    mul_rescale<KE>(s, leaf_tile(g0, LS), E);
   }
   // s = s o o (rescaled)
   __device__ __forceinline__ void mulpop(Tile& s, uint64_t& k, const Tile& o, uint64_t ko) {
     k &= ko;
-    mul_rescale(s, o, E);
+    mul_rescale<KE>(s, o, E);
   }
   // s = P s for the P tile staged in LDS; prefetches the tile of branch
   // next_br (or the next task's first tile) and, if pf_leaf names one, the
@@ -325,7 +341,7 @@ struct Walk {
       if (tid < 16) pf[PER] = *reinterpret_cast<const f32x4*>(nsrc + 4096 + tid * 4);
     }
     const float* Ab = Pbuf[kg & 1];
-    f32x4 acc[4];
+    f32x4 acc[4], accx[4];
     if constexpr (PREC == kPrecF32) {
 #pragma unroll
       for (int mo = 0; mo < 4; ++mo) acc[mo] = (f32x4)(0.0f);
@@ -344,18 +360,32 @@ struct Walk {
         }
       }
     } else {
-      gemm_half<PREC>(acc, s, Ab, lane);
+      gemm_half<PREC>(acc, accx, s, Ab, lane);
     }
     // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, in round-to-nearest VALU
+    // (for FP16 operands the product carries P's 2^14, taken off in the FMA)
 #pragma unroll
     for (int mo = 0; mo < 4; ++mo) {
       const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[mo][v] = s.r[mo][v] + fmaf(dd[v], s.r[mo][v], acc[mo][v]);
+      for (int v = 0; v < 4; ++v) {
+        if constexpr (PREC == kPrecF16S)
+          acc[mo][v] = fmaf(fmaf(accx[mo][v], kLoUnscale, acc[mo][v]), kF16Unscale,
+                            fmaf(dd[v], s.r[mo][v], s.r[mo][v]));
+        else if constexpr (KE != 0)
+          acc[mo][v] = fmaf(acc[mo][v], kF16Unscale, fmaf(dd[v], s.r[mo][v], s.r[mo][v]));
+        else
+          acc[mo][v] = s.r[mo][v] + fmaf(dd[v], s.r[mo][v], acc[mo][v]);
+      }
     }
-    const bool cst = (k >> lane) & 1;
+    if (k == 0) {  // no lane of this wave holds a constant column (k is wave-uniform)
 #pragma unroll
-    for (int mo = 0; mo < 4; ++mo) s.r[mo] = cst ? s.r[mo] : acc[mo];
+      for (int mo = 0; mo < 4; ++mo) s.r[mo] = acc[mo];
+    } else {
+      const bool cst = (k >> lane) & 1;
+#pragma unroll
+      for (int mo = 0; mo < 4; ++mo) s.r[mo] = cst ? s.r[mo] : acc[mo];
+    }
     if (nsrc) {
       float* dst = Pbuf[(kg & 1) ^ 1];
 #pragma unroll
