@@ -922,6 +922,12 @@ int score_mle(pcsf_batch* b, int precision, std::vector<PairState>& st) {
       return;
     }
     s.br.init(s.x, fmin, lo, flo, hi, fhi, fv);
+    // With FP32 / FP16 likelihoods (relative rounding noise ~1e-8, i.e. ~1e-5
+    // nats on a lpr of 1000) a step of sqrt(DBL_EPSILON)|x| changes lpr by
+    // less than that noise, and GSL's comparison of f(x +- tol) with f(x)
+    // would be decided by rounding; such steps are 1e-5 |x| instead, still
+    // far inside the maximiser's 1% stopping interval.
+    if (precision != PCSF_FP64) s.br.rel_tol = 1e-5;
     s.br.propose();
     s.pending = s.br.u;
     s.phase = PairState::BRENT;

@@ -120,7 +120,7 @@ void BrentMin::init(double xmin, double fmin, double lo, double flo, double hi, 
 double BrentMin::golden_point() const { return x_lo + kGolden * (x_hi - x_lo); }
 
 void BrentMin::propose() {
-  const double tol = 1.4901161193847656e-08 * std::fabs(x_min);  // GSL_SQRT_DBL_EPSILON
+  const double tol = rel_tol * std::fabs(x_min);  // GSL: GSL_SQRT_DBL_EPSILON * |x|
   const double mid = 0.5 * (x_lo + x_hi);
   const double z = x_min;
   // GSL keeps the last two steps in (d, e) and swaps them on entry

@@ -34,6 +34,10 @@ struct BrentMin {
   double x_min, f_min, x_lo, f_lo, x_hi, f_hi;
   double v, w, f_v, f_w, d, e;
   double u;  // pending abscissa
+  // smallest step relative to |x_min|: GSL's sqrt(DBL_EPSILON); callers whose
+  // function values carry more rounding noise than a step of that size can
+  // resolve (FP32 / FP16 likelihoods) raise it
+  double rel_tol = 1.4901161193847656e-08;
 
   // gsl_min_fminimizer_set_with_values + brent_init with f(v) already known
   void init(double xmin, double fmin, double lo, double flo, double hi, double fhi,

@@ -20,11 +20,11 @@ constexpr int kLeafRows = 65;  // leaf messages per branch: codons 0..63, 64 = `
 // (region, model) of a chunk of regions).
 // Formats of the P tile (PSets::d_Pfrag, kTileFloats floats per branch):
 //   kFragF32   off-diagonal FP32 in v_mfma_f32_16x16x4f32 A order + diagonal
-//   kFragF16   off-diagonal x 2^14 as FP16 hi, then the FP16 residual x 2^11
+//   kFragF16   (P - I) x 2^14 as FP16 hi, then the FP16 residual x 2^11
 //              (lo), in v_mfma_f32_16x16x32_f16 A order (4096 halves each)
 //              + diagonal
 //   kFragBF16  the same in BF16, unscaled
-// The diagonal part is always 64 FP32 values P[a][a] - 1 at float 4096.
+// The 64 FP32 values P[a][a] - 1 follow at float 4096 in every format.
 constexpr int kFragF32 = 0, kFragF16 = 1, kFragBF16 = 2;
 int frag_format(int precision);  // -1 for a precision without MFMA tiles
 

@@ -14,12 +14,11 @@
 //         P[a][c] / P[c][c] with lnPcc[c] = ln P[c][c] in FP64 (the dominant
 //         entry of a leaf message is exactly 1 and its value travels in FP64);
 //         row 64 is the all-ones message of `Marginalize (lnPcc 0).
-//   Pfrag tile of kTileFloats floats in format fmt: the off-diagonal of P
-//         in the A-operand order of the MFMA the kernel uses (diagonal
-//         entries zero) -- FP32 for v_mfma_f32_16x16x4f32, or hi and lo
-//         halves (FP16 scaled by 2^14, or BF16) for
-//         v_mfma_f32_16x16x32_{f16,bf16} -- followed by the 64 FP32 values
-//         P[a][a] - 1.
+//   Pfrag tile of kTileFloats floats in format fmt, in the A-operand order of
+//         the MFMA the kernel uses: FP32 off-diagonal of P (diagonal entries
+//         zero) for v_mfma_f32_16x16x4f32, or hi and lo halves of P - I (FP16
+//         scaled by 2^14, or BF16) for v_mfma_f32_16x16x32_{f16,bf16};
+//         followed by the 64 FP32 values P[a][a] - 1 (read by the FP32 path).
 #include "../internal.hpp"
 
 namespace {
@@ -130,7 +129,9 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
       const int j = e & 7, lane = (e >> 3) & 63, st = (e >> 9) & 1, mo = e >> 10;
       const int row = 16 * mo + (lane & 15);
       const int col = 16 * (2 * st + (j >> 2)) + 4 * (lane >> 4) + (j & 3);
-      const double x = (row == col) ? 0.0 : Vs[row * 64 + col] * sc;
+      // the half-precision tiles hold P - I (the diagonal term rides in the
+      // product, which starts from zero and stays small next to alpha)
+      const double x = ((row == col) ? Vs[row * 64 + col] - 1.0 : Vs[row * 64 + col]) * sc;
       if (fmt == kFragF16) {
         _Float16* h = reinterpret_cast<_Float16*>(Pfrag + fbase);
         const _Float16 hi = (_Float16)(float)x;

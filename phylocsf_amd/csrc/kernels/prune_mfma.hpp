@@ -52,8 +52,9 @@
 //    hi*hi in one accumulator and hi*lo + lo*hi in another: about 22
 //    significant bits per product at three FP16 MFMAs, against sixteen FP32
 //    MFMAs for the same 64x64x16 product.
-//    Live partials stay FP32 in registers in every precision; the diagonal
-//    of P stays in the FP32 VALU term.
+//    Live partials stay FP32 in registers in every precision.  The FP32 path
+//    keeps the diagonal of P in an FP32 VALU term; the half-precision tiles
+//    hold P - I, so their product is the whole (small) update of alpha.
 #include <type_traits>
 
 #include "../internal.hpp"
@@ -362,20 +363,27 @@ struct Walk {
     } else {
       gemm_half<PREC>(acc, accx, s, Ab, lane);
     }
-    // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, in round-to-nearest VALU
-    // (for FP16 operands the product carries P's 2^14, taken off in the FMA)
+    // FP32: m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, the diagonal
+    // term in round-to-nearest VALU arithmetic; half precisions: the product
+    // is (P - I) alpha (carrying P's 2^14 for FP16, taken off in the FMA)
 #pragma unroll
     for (int mo = 0; mo < 4; ++mo) {
-      const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
+      if constexpr (PREC == kPrecF32) {
+        const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        if constexpr (PREC == kPrecF16S)
-          acc[mo][v] = fmaf(fmaf(accx[mo][v], kLoUnscale, acc[mo][v]), kF16Unscale,
-                            fmaf(dd[v], s.r[mo][v], s.r[mo][v]));
-        else if constexpr (KE != 0)
-          acc[mo][v] = fmaf(acc[mo][v], kF16Unscale, fmaf(dd[v], s.r[mo][v], s.r[mo][v]));
-        else
+        for (int v = 0; v < 4; ++v)
           acc[mo][v] = s.r[mo][v] + fmaf(dd[v], s.r[mo][v], acc[mo][v]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          if constexpr (PREC == kPrecF16S)
+            acc[mo][v] =
+                fmaf(fmaf(accx[mo][v], kLoUnscale, acc[mo][v]), kF16Unscale, s.r[mo][v]);
+          else if constexpr (KE != 0)
+            acc[mo][v] = fmaf(acc[mo][v], kF16Unscale, s.r[mo][v]);
+          else
+            acc[mo][v] = s.r[mo][v] + acc[mo][v];
+        }
       }
     }
     if (k == 0) {  // no lane of this wave holds a constant column (k is wave-uniform)
