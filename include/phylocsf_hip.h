@@ -156,7 +156,11 @@ int pcsf_forward(pcsf_batch *b, const double *rho, int rho_per_region, int preci
  * posteriors of PhyloLik.ml:140-174, equal to d ln z/dt), computed by the
  * outside (preorder) pass of PhyloLik.ensure_beta (PhyloLik.ml:96-120) in
  * FP64; the inside pass is recomputed, so forward saves no partials.
- * Per-branch sums use FP64 atomics (not bitwise reproducible run to run). */
+ * Per-branch sums use FP64 atomics (not bitwise reproducible run to run).
+ * After a forward at an MFMA precision (PCSF_FP32 .. PCSF_BF16) and with
+ * grad_branch NULL, dL/drho comes from a forward-mode (tangent) pass of the
+ * fused kernel in FP32 instead: d a_v/d rho propagated beside a_v through
+ * D = b Q P(rho b) on every branch. */
 int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
                   double *grad_branch);
 

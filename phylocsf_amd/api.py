@@ -309,11 +309,14 @@ class Batch:
         el = np.zeros((self.n_regions, nm))
         self.model.ctx.check(lib().pcsf_forward(self.h, _dp(r), 1 if per_region else 0,
                                                 precision, _dp(ll), _dp(el), 0))
+        self.forward_count = getattr(self, "forward_count", 0) + 1
         return ll, el
 
     def backward(self, grad_loglik, branches=False):
         """Gradients of the last forward: -> (dL/drho [R, nm], dL/dt_br [R, nm, n-1] or None)
-        for dL/dloglik = grad_loglik [R, nm] (outside pass, FP64)."""
+        for dL/dloglik = grad_loglik [R, nm]: FP64 outside pass, or the FP32
+        tangent pass when the last forward ran at an MFMA precision and
+        branches is False."""
         nm = self.model.nm
         g = np.ascontiguousarray(np.broadcast_to(np.asarray(grad_loglik, dtype=np.float64),
                                                  (self.n_regions, nm)))
@@ -364,6 +367,8 @@ def score(batch: Batch, strategy="fixed", precision=PCSF_FP32):
     out = dict(score=np.zeros(R), anc_comp=np.zeros(R), rho=np.zeros((R, 2)),
                lpr=np.zeros((R, 2)), elpr=np.zeros((R, 2)),
                status=np.zeros(R, np.int32), n_evals=np.zeros(R, np.int32))
+    # the maximisers run forwards of their own on this batch
+    batch.forward_count = getattr(batch, "forward_count", 0) + 1
     batch.model.ctx.check(lib().pcsf_score(batch.h, st, precision, _dp(out["score"]),
                                            _dp(out["anc_comp"]), _dp(out["rho"]),
                                            _dp(out["lpr"]), _dp(out["elpr"]),

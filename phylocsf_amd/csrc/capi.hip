@@ -43,7 +43,8 @@ void dfree(T*& p) {
 
 bool rho_valid(double r) { return r > 0.0 && std::isfinite(r); }
 
-hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0) {
+hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0,
+                       bool tangent = false) {
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) {
     if (e == hipSuccess) e = x;
@@ -55,6 +56,10 @@ hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0) {
   chk(dalloc(&ps.d_rho, (size_t)cap));
   chk(dalloc(&ps.d_active, (size_t)cap));
   if (fp64) chk(dalloc(&ps.d_P64T, n * 4096));
+  if (tangent) {
+    chk(dalloc(&ps.d_DT32, n * kLeafRows * 64));
+    chk(dalloc(&ps.d_Dfrag, n * 4096));
+  }
   if (nctab > 0) {
     chk(dalloc(&ps.d_ctab, (size_t)cap * nctab * kTabCodes * kTabCodes * 64));
     chk(dalloc(&ps.d_clnt, (size_t)cap * nctab * kTabCodes * kTabCodes));
@@ -73,6 +78,8 @@ void free_psets(PSets& ps) {
   dfree(ps.d_active);
   dfree(ps.d_ctab);
   dfree(ps.d_clnt);
+  dfree(ps.d_DT32);
+  dfree(ps.d_Dfrag);
   ps.cap = 0;
 }
 
@@ -184,9 +191,9 @@ int ensure_aligned(pcsf_batch* b) {
 }
 
 // Bytes of one transition set on the device.
-size_t set_bytes(const pcsf_model* m, bool fp64) {
-  return (size_t)m->nbr *
-         (kLeafRows * 64 * 4 + kTileFloats * 4 + kLeafRows * 8 + (fp64 ? 4096 * 8 : 0));
+size_t set_bytes(const pcsf_model* m, bool fp64, bool tangent = false) {
+  return (size_t)m->nbr * (kLeafRows * 64 * 4 + kTileFloats * 4 + kLeafRows * 8 +
+                           (fp64 ? 4096 * 8 : 0) + (tangent ? (kLeafRows + 64) * 64 * 4 : 0));
 }
 
 // Forward with one tree scale per (region, model): rho[r*nm+q].  active
@@ -264,22 +271,25 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
   return PCSF_OK;
 }
 
-// Adjoint pass at tree scales rho (nreg * nm) over the pairs in `active`
+// Gradient pass at tree scales rho (nreg * nm) over the pairs in `active`
 // (optional): per-region lpr into d_loglik and d lpr / d rho into d_elpr
-// (the backward reuses the elpr buffers), and, if d_gbr is given, the
-// per-region sums of d ln z / d t_br (nreg * nm * nbr, accumulated).
-int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, double* d_gbr) {
+// (the backward reuses the elpr buffers).  tangent = false: the FP64
+// outside-pass adjoint, which also accumulates, if d_gbr is given, the
+// per-region sums of d ln z / d t_br (nreg * nm * nbr); tangent = true: the
+// FP32 MFMA tangent pass (d lpr / d rho only).
+int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, double* d_gbr,
+                    bool tangent = false) {
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
   const int nm = m->nm;
   int rc = ensure_aligned(b);
   if (rc) return rc;
   const size_t budget = (size_t)8 << 30;
-  int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, true) / nm * nm);
+  int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, !tangent, tangent) / nm * nm);
   cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
-  if (b->rs.cap < cap || !b->rs.fp64) {
+  if (b->rs.cap < cap || (!tangent && !b->rs.fp64) || (tangent && !b->rs.d_Dfrag)) {
     free_psets(b->rs);
-    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, true));
+    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, !tangent, 0, tangent));
   }
   const int chunk_regions = b->rs.cap / nm;
   std::vector<uint8_t> act;
@@ -307,9 +317,11 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
     float tp = 0.f;
     rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, kFragF32, &tp);
     if (rc) return rc;
-    hipError_t e = launch_adjoint_fp64(
-        m, b, ch, d_gbr ? d_gbr + (size_t)r0 * nm * m->nbr : nullptr, c->stream);
-    if (e == hipErrorInvalidValue)
+    hipError_t e = tangent ? launch_prune_tangent_regions(m, b, ch, c->stream)
+                           : launch_adjoint_fp64(
+                                 m, b, ch, d_gbr ? d_gbr + (size_t)r0 * nm * m->nbr : nullptr,
+                                 c->stream);
+    if (e == hipErrorInvalidValue && !tangent)
       return set_err(c, PCSF_EINVAL, "adjoint kernel: tree workspace exceeds 64 KiB of LDS");
     HIP_TRY(c, e);
     HIP_TRY(c, launch_region_reduce_aligned(b, ch, c->stream));
@@ -519,6 +531,25 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   HIP_TRY(c, dalloc(&m->d_prog, m->prog.size()));
   HIP_TRY(c, dalloc(&m->d_prog_t, m->prog_t.size()));
   HIP_TRY(c, dalloc(&m->d_cherries, std::max<size_t>(1, m->cherries.size())));
+  // leaves each program gathers (CHERRY operands, LEAFMUL operands)
+  auto gathered = [&](const std::vector<int32_t>& ops) {
+    std::vector<int16_t> out;
+    for (size_t k = 0; k + kOpWords <= ops.size(); k += kOpWords) {
+      const int op = ops[k] & 0xff, a0 = ops[k + 1] & 0xffff, a1 = (ops[k + 1] >> 16) & 0xffff;
+      if (op == OP_CHERRY) {
+        out.push_back((int16_t)a0);
+        out.push_back((int16_t)a1);
+      } else if (op == OP_LEAFMUL) {
+        out.push_back((int16_t)a0);
+      }
+    }
+    return out;
+  };
+  const std::vector<int16_t> lsl = gathered(m->prog), lsl_t = gathered(m->prog_t);
+  m->n_lsl = (int)lsl.size();
+  m->n_lsl_t = (int)lsl_t.size();
+  HIP_TRY(c, dalloc(&m->d_lsl, std::max<size_t>(1, lsl.size())));
+  HIP_TRY(c, dalloc(&m->d_lsl_t, std::max<size_t>(1, lsl_t.size())));
   HIP_TRY(c, dalloc(&m->d_perr, (size_t)8));
   HIP_TRY(c, dalloc(&m->d_perrval, (size_t)1));
   HIP_TRY(c, alloc_psets(m->fixed, nm, m->nbr, true, (int)m->cherries.size() / 3));
@@ -539,6 +570,8 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   HIP_TRY(c, up(m->d_prog_t, m->prog_t.data(), m->prog_t.size() * 4));
   if (!m->cherries.empty())
     HIP_TRY(c, up(m->d_cherries, m->cherries.data(), m->cherries.size() * 4));
+  if (!lsl.empty()) HIP_TRY(c, up(m->d_lsl, lsl.data(), lsl.size() * 2));
+  if (!lsl_t.empty()) HIP_TRY(c, up(m->d_lsl_t, lsl_t.data(), lsl_t.size() * 2));
   *out = guard.release();
   return PCSF_OK;
 }
@@ -557,6 +590,8 @@ void pcsf_model_destroy(pcsf_model* m) {
   dfree(m->d_prog);
   dfree(m->d_prog_t);
   dfree(m->d_cherries);
+  dfree(m->d_lsl);
+  dfree(m->d_lsl_t);
   dfree(m->d_perr);
   dfree(m->d_perrval);
   free_psets(m->fixed);
@@ -736,6 +771,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
     int rc = forward_regions(b, rho, nullptr, precision);
     if (rc) return rc;
     b->last_rho.assign(rho, rho + nr);
+    b->last_precision = precision;
     if (loglik && nr)
       HIP_TRY(c, hipMemcpyAsync(loglik, b->d_loglik, nr * 8, hipMemcpyDeviceToHost, c->stream));
     if (elpr && nr)
@@ -756,6 +792,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   if (rc) return rc;
   b->t_pbuild = tp;
   b->last_rho.resize((size_t)b->nreg * m->nm);
+  b->last_precision = precision;
   for (int32_t i = 0; i < b->nreg; ++i)
     for (int q = 0; q < m->nm; ++q) b->last_rho[(size_t)i * m->nm + q] = r[q];
   if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
@@ -1061,7 +1098,8 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   std::vector<double> rho(np, 1.0), L(np), D(np);
   std::vector<uint8_t> act(np, 1);
   auto eval = [&]() -> int {
-    int rc = adjoint_regions(b, rho.data(), act.data(), nullptr);
+    // FP64: the outside-pass adjoint; MFMA precisions: the FP32 tangent pass
+    int rc = adjoint_regions(b, rho.data(), act.data(), nullptr, precision != PCSF_FP64);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(D.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost));
@@ -1219,7 +1257,10 @@ int pcsf_backward(pcsf_batch* b, const double* grad_loglik, double* grad_rho,
     HIP_TRY(c, dalloc(&d_gbr, ngb));
     HIP_TRY(c, hipMemsetAsync(d_gbr, 0, ngb * 8, c->stream));
   }
-  int rc = adjoint_regions(b, b->last_rho.data(), nullptr, d_gbr);
+  // an MFMA-precision forward without per-branch gradients is differentiated
+  // by the FP32 tangent pass; everything else by the FP64 outside pass
+  const bool tangent = b->last_precision != PCSF_FP64 && !d_gbr;
+  int rc = adjoint_regions(b, b->last_rho.data(), nullptr, d_gbr, tangent);
   std::vector<double> dl(np), gb;
   if (!rc && np) {
     hipError_t e = hipMemcpy(dl.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost);

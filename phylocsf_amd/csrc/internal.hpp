@@ -20,12 +20,16 @@ constexpr int kLeafRows = 65;  // leaf messages per branch: codons 0..63, 64 = `
 // (region, model) of a chunk of regions).
 // Formats of the P tile (PSets::d_Pfrag, kTileFloats floats per branch):
 //   kFragF32   off-diagonal FP32 in v_mfma_f32_16x16x4f32 A order + diagonal
-//   kFragF16   (P - I) x 2^14 as FP16 hi, then the FP16 residual x 2^11
-//              (lo), in v_mfma_f32_16x16x32_f16 A order (4096 halves each)
-//              + diagonal
-//   kFragBF16  the same in BF16, unscaled
+//   kFragF16   x = (P - I) x 2^13 as FP16 hi = RN(x), then the FP16 residual
+//              lo = RN((x - hi) x 2^13), in v_mfma_f32_16x16x32_f16 A order
+//              (4096 halves each) + diagonal
+//   kFragBF16  the same in BF16, unscaled, residual unscaled
 // The 64 FP32 values P[a][a] - 1 follow at float 4096 in every format.
 constexpr int kFragF32 = 0, kFragF16 = 1, kFragBF16 = 2;
+// 2^kHalfExp: the scale of the FP16 P tiles and of their residuals.  2^13
+// keeps |x| < 2^13 and |lo| <= 2^14 inside FP16's range and their entries
+// above 2^-27 normal numbers (the matrix cores read FP16 subnormals as zero).
+constexpr int kHalfExp = 13;
 int frag_format(int precision);  // -1 for a precision without MFMA tiles
 
 struct PSets {
@@ -38,6 +42,9 @@ struct PSets {
   float* d_Pfrag = nullptr;    // cap * nbr * kTileFloats: off-diagonal of P in MFMA
                                //   A-fragment order, then the 64 values P[a][a] - 1
   double* d_lnPcc = nullptr;   // cap * nbr * 65 FP64 ln P[c][c] (0 for row 64)
+  // tangent data (allocated on request): D = dP/drho = b Q P(rho b)
+  float* d_DT32 = nullptr;     // cap * nbr * 65 * 64: row c < 64 D[a][c] / P[c][c], row 64 0
+  float* d_Dfrag = nullptr;    // cap * nbr * 4096: D in the FP32 A-operand order
   double* d_rho = nullptr;     // cap tree scales
   uint8_t* d_active = nullptr; // cap flags: 0 = set not needed by this launch
   // cherry tables (fixed scales only): message P_c (P_a[:,x] o P_b[:,y]) of
@@ -88,6 +95,10 @@ struct pcsf_model {
   int32_t* d_lc = nullptr;     // n
   int32_t* d_rc = nullptr;
   int32_t* d_prog = nullptr;
+  // leaves gathered by prog / prog_t (the others are read through tables)
+  int16_t* d_lsl = nullptr;
+  int16_t* d_lsl_t = nullptr;
+  int n_lsl = 0, n_lsl_t = 0;
   // device: transition matrices at the cached tree scales (set q = model q)
   std::vector<double> cached_rho;
   int cached_fmt = -1;
@@ -123,6 +134,7 @@ struct pcsf_batch {
   std::vector<int64_t> offsets;     // host copy
   Aligned al;                       // built on the first per-region forward
   std::vector<double> last_rho;     // tree scales of the last forward (nreg * nm)
+  int last_precision = 0;           // PCSF_FP64 .. PCSF_BF16 of the last forward
   PSets rs;                         // per-region transition sets (one chunk)
   double* d_alogz = nullptr;        // nm * al.ld, per aligned column
   double* d_acolelpr = nullptr;
@@ -153,6 +165,8 @@ hipError_t launch_frames(const uint8_t* nuc, int64_t nlen, const int64_t* d_roff
                          int64_t ld, hipStream_t s);
 hipError_t launch_adjoint_fp64(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
                                double* grad_branch, hipStream_t s);
+hipError_t launch_prune_tangent_regions(const pcsf_model* m, const pcsf_batch* b,
+                                        const RegionChunk& ch, hipStream_t s);
 hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& ch,
                                         hipStream_t s);
 // Launch shapes of the fused kernel that the autotuner chooses among.

@@ -17,7 +17,7 @@
 //   Pfrag tile of kTileFloats floats in format fmt, in the A-operand order of
 //         the MFMA the kernel uses: FP32 off-diagonal of P (diagonal entries
 //         zero) for v_mfma_f32_16x16x4f32, or hi and lo halves of P - I (FP16
-//         scaled by 2^14, or BF16) for v_mfma_f32_16x16x32_{f16,bf16};
+//         scaled by 2^kHalfExp, or BF16) for v_mfma_f32_16x16x32_{f16,bf16};
 //         followed by the 64 FP32 values P[a][a] - 1 (read by the FP32 path).
 #include "../internal.hpp"
 
@@ -30,7 +30,8 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
     const double* __restrict__ lam, const double* __restrict__ branch,
     const double* __restrict__ rho, const uint8_t* __restrict__ active, int nm, int nbr,
     double* __restrict__ P64T, float* __restrict__ PT32, float* __restrict__ Pfrag, int fmt,
-    double* __restrict__ lnPcc, int* __restrict__ perr, double* __restrict__ perrval) {
+    double* __restrict__ lnPcc, int* __restrict__ perr, double* __restrict__ perrval,
+    const double* __restrict__ QT, float* __restrict__ DT32, float* __restrict__ Dfrag) {
   __shared__ double W[64 * 64];  // diag(exp(lam t)) V^-1, then P
   __shared__ double Vs[64 * 64];
   const int br = blockIdx.x, set = blockIdx.y, q = set % nm, tid = threadIdx.x;
@@ -122,9 +123,9 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
     //   e = ((mo*2 + st)*64 + lane)*8 + j
     //   row = 16*mo + lane%16, col = 16*(2*st + j/4) + 4*(lane/16) + j%4
     // hi = x rounded to nearest, lo = the residual x - hi rounded to nearest;
-    // FP16: x = P * 2^14 and lo carries a further 2^11 (a normal number
+    // FP16: x = (P - I) 2^13 and lo carries a further 2^13 (a normal number
     // wherever hi is: the matrix cores read FP16 subnormals as zero)
-    const double sc = (fmt == kFragF16) ? 16384.0 : 1.0;
+    const double sc = (fmt == kFragF16) ? (double)(1 << kHalfExp) : 1.0;
     for (int e = tid; e < 4096; e += 256) {
       const int j = e & 7, lane = (e >> 3) & 63, st = (e >> 9) & 1, mo = e >> 10;
       const int row = 16 * mo + (lane & 15);
@@ -134,18 +135,53 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
       const double x = ((row == col) ? Vs[row * 64 + col] - 1.0 : Vs[row * 64 + col]) * sc;
       if (fmt == kFragF16) {
         _Float16* h = reinterpret_cast<_Float16*>(Pfrag + fbase);
-        const _Float16 hi = (_Float16)(float)x;
+        const _Float16 hi = (_Float16)x;
         h[e] = hi;
-        h[4096 + e] = (_Float16)(float)((x - (double)(float)hi) * 2048.0);
+        h[4096 + e] = (_Float16)((x - (double)hi) * sc);
       } else {
         __bf16* h = reinterpret_cast<__bf16*>(Pfrag + fbase);
-        const __bf16 hi = (__bf16)(float)x;
+        const __bf16 hi = (__bf16)x;
         h[e] = hi;
-        h[4096 + e] = (__bf16)(float)(x - (double)(float)hi);
+        h[4096 + e] = (__bf16)(x - (double)hi);
       }
     }
   }
   if (tid < 64) Pfrag[fbase + 4096 + tid] = (float)(Vs[tid * 64 + tid] - 1.0);
+  if (!Dfrag) return;
+  // D = dP(rho b)/drho = b Q P(t) for the tangent kernel: leaf rows
+  // D[a][c] / P[c][c] (row 64 zero: `Marginalize does not depend on rho) and
+  // the dense FP32 tile in the same A-operand order as Pfrag
+  __syncthreads();
+  const double* Qq = QT + (size_t)q * 4096;  // QT[k*64 + a] = Q[a][k]
+  for (int e = tid; e < 4096; e += 256) W[(e & 63) * 64 + (e >> 6)] = Qq[e];
+  __syncthreads();
+  double dacc[4][4] = {};
+  for (int k = 0; k < 64; ++k) {
+    double a[4], p[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = W[(r0 + i) * 64 + k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p[j] = Vs[k * 64 + c0 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dacc[i][j] = fma(a[i], p[j], dacc[i][j]);
+  }
+  __syncthreads();  // every thread is done reading Q from W
+  const double bl = branch[br];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) W[(r0 + i) * 64 + c0 + j] = bl * dacc[i][j];
+  __syncthreads();
+  for (int e = tid; e < 4096; e += 256) {
+    const int c = e >> 6, a = e & 63;
+    DT32[lbase + e] = (float)(W[a * 64 + c] / Vs[c * 64 + c]);
+    const int v = e & 3, lane = (e >> 2) & 63, mb = (e >> 8) & 3, mo = e >> 10;
+    Dfrag[((size_t)set * nbr + br) * 4096 + e] =
+        (float)W[(16 * mo + (lane & 15)) * 64 + 16 * mb + 4 * (lane >> 4) + v];
+  }
+  if (tid < 64) DT32[lbase + 4096 + tid] = 0.0f;
 }
 
 // Cherry tables: for cherry c with leaves a, b (branches into them from c),
@@ -201,6 +237,7 @@ hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int 
   hipLaunchKernelGGL(pbuild_kernel, grid, dim3(256), 0, s, m->d_V, m->d_Vinv, m->d_lam,
                      m->d_branch, sets.d_rho, sets.d_active, m->nm, m->nbr,
                      sets.fp64 ? sets.d_P64T : nullptr, sets.d_PT32, sets.d_Pfrag, fmt,
-                     sets.d_lnPcc, m->d_perr, m->d_perrval);
+                     sets.d_lnPcc, m->d_perr, m->d_perrval, m->d_QT, sets.d_DT32,
+                     sets.d_Dfrag);
   return hipGetLastError();
 }

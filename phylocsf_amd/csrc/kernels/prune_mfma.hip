@@ -47,7 +47,11 @@ MfmaParams base_params(const pcsf_model* m, const PSets& sets) {
   prm.nl = m->nl;
   prm.nm = m->nm;
   prm.nbr = m->nbr;
+#ifdef PCSF_XNOTAB
+  const bool tables = false;
+#else
   const bool tables = sets.d_ctab != nullptr && !m->cherries.empty();
+#endif
   prm.first_gemm_br = tables ? m->first_gemm_br_t : m->first_gemm_br;
   prm.prog = tables ? m->d_prog_t : m->d_prog;
   prm.ctab = tables ? sets.d_ctab : nullptr;
@@ -58,6 +62,8 @@ MfmaParams base_params(const pcsf_model* m, const PSets& sets) {
   prm.lnPcc = sets.d_lnPcc;
   prm.pi = m->d_pi;
   prm.lnpi = m->d_lnpi;
+  prm.ls_leaves = tables ? m->d_lsl_t : m->d_lsl;
+  prm.n_ls = tables ? m->n_lsl_t : m->n_lsl;
   return prm;
 }
 }  // namespace

@@ -41,17 +41,18 @@
 //    L2 holds one model's matrices.
 //  * GEMM precision (template PREC): FP32 on v_mfma_f32_16x16x4f32; or P and
 //    the partial enter v_mfma_f32_16x16x32_{f16,bf16} in half precision with
-//    FP32 accumulation.  FP16 operands are scaled by 2^14 (P's off-diagonal
-//    entries are below 1; the live tiles of the FP16 kernels are kept with
-//    their column maximum in [2^13, 2^14) by the rescaling itself, so nothing
-//    overflows and values down to 2^-28 of the maximum stay normal) and P's
-//    2^14 is taken off the product exactly.  The split-FP16 precision carries each
-//    operand as hi + lo, lo = the FP16 rounding of the residual scaled by 2^11
-//    (so that it stays a normal number: the matrix cores treat FP16
-//    subnormals as zero, which measurably biased an unscaled lo), and forms
-//    hi*hi in one accumulator and hi*lo + lo*hi in another: about 22
-//    significant bits per product at three FP16 MFMAs, against sixteen FP32
-//    MFMAs for the same 64x64x16 product.
+//    FP32 accumulation.  FP16 tiles hold (P - I) 2^13 (entries below 2^13 in
+//    magnitude); the live tiles of the FP16 kernels are kept with their
+//    column maximum in [2^25, 2^26) by the rescaling itself and enter the
+//    GEMM as RN(s 2^-13), maximum in [2^12, 2^13): nothing overflows, values
+//    down to 2^-26 of the maximum stay normal, and the product comes out at
+//    the tile's own scale.  The split-FP16 precision carries each operand as
+//    hi + lo, hi rounded to nearest and lo = the FP16 rounding of the
+//    residual scaled by 2^13 (so that it stays a normal number: the matrix
+//    cores treat FP16 subnormals as zero, which measurably biased an
+//    unscaled lo), and forms hi*hi in one accumulator and hi*lo + lo*hi in
+//    another: about 23 significant bits per product at three FP16 MFMAs,
+//    against sixteen FP32 MFMAs for the same 64x64x16 product.
 //    Live partials stay FP32 in registers in every precision.  The FP32 path
 //    keeps the diagonal of P in an FP32 VALU term; the half-precision tiles
 //    hold P - I, so their product is the whole (small) update of alpha.
@@ -68,10 +69,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // precisions of the GEMM (values of PCSF_FP32 .. PCSF_BF16)
 constexpr int kPrecF32 = PCSF_FP32, kPrecF16S = PCSF_FP16S, kPrecF16 = PCSF_FP16,
               kPrecBF16 = PCSF_BF16;
-constexpr int kF16Exp = 14;                 // FP16 operands are scaled by 2^14
-constexpr float kF16Unscale = 1.0f / 16384.0f;  // P's scale, taken off the product
-constexpr float kLoScale = 2048.0f;             // lo parts of split FP16 operands: x 2^11
-constexpr float kLoUnscale = 1.0f / 2048.0f;
+constexpr float kHalfUnscale = 1.0f / (float)(1 << kHalfExp);  // off the cross terms
 
 struct MfmaParams {
   const uint8_t* codes;
@@ -94,6 +92,13 @@ struct MfmaParams {
   const uint8_t* active;        // per set: 0 = skip
   int r0;                       // first region of the chunk
   int64_t g0;                   // first group of the chunk
+  // tangent kernel only: D = dP/drho tiles and leaf rows (prune_tangent.hip)
+  const float* Dfrag;
+  const float* DT32;
+  // leaves whose message is gathered (not read through a cherry table): their
+  // ln P[c][c] are summed once per task, before the walk
+  const int16_t* ls_leaves;
+  int n_ls;
 };
 
 struct Task {
@@ -143,55 +148,118 @@ struct Tile {
   f32x4 r[4];
 };
 
+// Raw buffer loads: a wave-uniform base in a buffer resource (scalar
+// registers) plus a 32-bit lane offset and a scalar offset, so the address
+// arithmetic of every table read is scalar but for one VGPR; reads past
+// `bytes` return zero instead of touching memory.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+#ifndef PCSF_XGLOBAL
+__device__ __forceinline__ f32x4 bload4(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ double bload_f64(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+#else
+struct XR { const char* b; };
+#define make_rsrc(base, bytes) XR{reinterpret_cast<const char*>(base)}
+#define Rsrc XR
+__device__ __forceinline__ f32x4 bload4(XR r, uint32_t voff, uint32_t soff) {
+  return *reinterpret_cast<const f32x4*>(r.b + soff + voff);
+}
+__device__ __forceinline__ double bload_f64(XR r, uint32_t voff, uint32_t soff) {
+  return *reinterpret_cast<const double*>(r.b + soff + voff);
+}
+#endif
+
 struct Gathered {
   Tile t;
-  double ln;   // ln P[c][c] (0 for `Marginalize)
   bool marg;
 };
 
 // Loads the leaf message of leaf `leaf` for this lane's codon: the
-// normalised column of P, or row 64 (all ones) for `Marginalize.
+// normalised column of P, or row 64 (all ones) for `Marginalize (its
+// ln P[c][c] is summed before the walk: leaf_log_sum).
 __device__ __forceinline__ void gather(Gathered& g, const MfmaParams& p, const uint8_t* codes_s,
                                        int gc, int set, int leaf, int ccol, int grp) {
   const int code = codes_s[leaf * gc + ccol];
   g.marg = code == PCSF_MARGINALIZE;
-  const int c = g.marg ? 64 : code;
-  const size_t bc = ((size_t)set * p.nbr + leaf) * kLeafRows + c;
-  const float* src = p.PT32 + bc * 64 + 4 * grp;
+  const uint32_t c = g.marg ? 64 : code;
+  constexpr uint32_t kRowsBytes = kLeafRows * 64 * 4;  // one branch's leaf rows
+  const Rsrc r = make_rsrc(p.PT32 + (size_t)set * p.nbr * (kLeafRows * 64), p.nbr * kRowsBytes);
+  const uint32_t off = c * 256u + (uint32_t)grp * 16u;
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb) g.t.r[mb] = *reinterpret_cast<const f32x4*>(src + 16 * mb);
-  g.ln = p.lnPcc[bc];
+  for (int mb = 0; mb < 4; ++mb) g.t.r[mb] = bload4(r, off + 64u * mb, leaf * kRowsBytes);
 }
 
-__device__ __forceinline__ const Tile& leaf_tile(Gathered& g, double& ls) {
-  ls += g.ln;
-  return g.t;
+// Sum of ln P[c][c] over the gathered leaves of this lane's column.
+__device__ __forceinline__ double leaf_log_sum(const MfmaParams& p, const uint8_t* codes_s,
+                                              int gc, int set, int ccol) {
+  double ls = 0.0;
+#pragma unroll 8
+  for (int i = 0; i < p.n_ls; ++i) {
+    const int leaf = p.ls_leaves[i];
+    const int code = codes_s[leaf * gc + ccol];
+    const int c = code == PCSF_MARGINALIZE ? 64 : code;
+    ls += p.lnPcc[((size_t)set * p.nbr + leaf) * kLeafRows + c];
+  }
+  return ls;
 }
+
+__device__ __forceinline__ const Tile& leaf_tile(Gathered& g) { return g.t; }
 
 // s *= o, then every column is scaled by the power of two that brings its
 // maximum to [2^(KE-1), 2^KE); E accumulates the exponents, so the true
 // product of the column is s * 2^E.
 template <int KE>
 __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
-  float mx = 0.0f;
+  // column maximum on the bit patterns: positive floats order as signed
+  // integers, and negative values (a rounding residue below zero) and -0
+  // compare below +0 as they should
+  int mx = 0;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) {
     s.r[mb] *= o.r[mb];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) mx = fmaxf(mx, s.r[mb][v]);
+    for (int v = 0; v < 4; ++v) mx = max(mx, __float_as_int(s.r[mb][v]));
   }
-  mx = fmaxf(mx, __shfl_xor(mx, 16));
-  mx = fmaxf(mx, __shfl_xor(mx, 32));
-  const int e = (mx > 0.0f) ? __builtin_amdgcn_frexp_expf(mx) : 0;
+  // lanes l, l^16, l^32, l^48 hold the column
+#ifdef PCSF_XPERM
+  {
+    const auto a = __builtin_amdgcn_permlane16_swap(mx, mx, false, false);
+    mx = max((int)a[0], (int)a[1]);
+    const auto b = __builtin_amdgcn_permlane32_swap(mx, mx, false, false);
+    mx = max((int)b[0], (int)b[1]);
+  }
+#else
+  mx = max(mx, __shfl_xor(mx, 16));
+  mx = max(mx, __shfl_xor(mx, 32));
+#endif
+  const int e = (mx > 0) ? __builtin_amdgcn_frexp_expf(__int_as_float(mx)) : 0;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
     for (int v = 0; v < 4; ++v) s.r[mb][v] = __builtin_amdgcn_ldexpf(s.r[mb][v], KE - e);
   E += e - KE;
+  // keep the running sum here: left to itself the scheduler of a
+  // straight-line program defers every exponent to the root, holding one
+  // register per product live across the whole tree
+  asm("" : "+v"(E));
 }
 
-// 64x64 (off-diagonal P, LDS tile Ab) times the 64x16 tile s in half
-// precision; acc = the product, unscaled.  Tile layout of the halves
+// acc = (P - I) s in half precision for the P - I tile staged in LDS (Ab)
+// and the 64x16 tile s; F16S leaves the cross terms hi*lo + lo*hi, which
+// carry 2^kHalfExp, in accx.  The tile is kept at 2^kHalfExp above its GEMM
+// operand (FP16: operand maximum in [2^12, 2^13)), so the product comes out
+// at the tile's scale, and hi = RN(s 2^-13) and the residual RN(s - 2^13 hi)
+// are one mixed-precision FMA each.  Both accumulators start from zero (the
+// matrix cores' FP32 accumulation rounds toward zero, which would bias s if
+// it were the initial value).  Columns flagged cst (wave-uniform any_cst: at
+// least one) take a zero operand, so their update is exactly zero.  Tile
+// layout of the halves
 // (pbuild.hip): hi fragments, then lo fragments, 512 x 8 values each, in the
 // A-operand order of v_mfma_f32_16x16x32_{f16,bf16} with K permuted so that
 // K-step st of a lane in group g takes rows 16*(2*st + j/4) + 4*g + j%4 of
@@ -199,26 +267,37 @@ __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
 // s.r[2*st + j/4][j%4].
 template <int PREC>
 __device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], f32x4 (&accx)[4], const Tile& s,
-                                          const float* Ab, int lane) {
+                                          const float* Ab, int lane, bool any_cst, bool cst) {
   constexpr bool kBF = PREC == kPrecBF16;
   using V8 = typename std::conditional<kBF, bf16x8, f16x8>::type;
   using E = typename std::conditional<kBF, __bf16, _Float16>::type;
+  constexpr float kDown = kBF ? 1.0f : 1.0f / (float)(1 << kHalfExp);
   V8 bh[2], bl[2];
 #pragma unroll
-  for (int st = 0; st < 2; ++st)
+  for (int st = 0; st < 2; ++st) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float x = s.r[2 * st + (j >> 2)][j & 3];
-      if constexpr (PREC == kPrecF16S) {
-        // hi = x cut to 11 significant bits (exact in FP16), lo = the exact
-        // remainder, scaled and rounded
-        const float h = __uint_as_float(__float_as_uint(x) & 0xFFFFE000u);
-        bh[st][j] = (E)h;
-        bl[st][j] = (E)((x - h) * kLoScale);
-      } else {
-        bh[st][j] = (E)x;
-      }
+    for (int j = 0; j < 8; ++j) bh[st][j] = (E)(s.r[2 * st + (j >> 2)][j & 3] * kDown);
+    if constexpr (PREC == kPrecF16S) {
+      // the residual is taken against the rounded hi in its register (the
+      // compiler would otherwise recompute hi inside each residual FMA)
+      asm("" : "+v"(bh[st]));
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        bl[st][j] = (E)fmaf((float)bh[st][j], -(float)(1 << kHalfExp),
+                            s.r[2 * st + (j >> 2)][j & 3]);
     }
+  }
+#ifdef PCSF_XSEL
+  if (false) {
+#else
+  if (any_cst) {
+#endif
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      bh[st] = cst ? (V8)(E)0 : bh[st];
+      if constexpr (PREC == kPrecF16S) bl[st] = cst ? (V8)(E)0 : bl[st];
+    }
+  }
   const V8* Ah = reinterpret_cast<const V8*>(Ab);
   const V8* Al = Ah + 512;
 #pragma unroll
@@ -227,22 +306,30 @@ __device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], f32x4 (&accx)[4], con
     if constexpr (kBF) return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     else return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   };
-  if constexpr (PREC == kPrecF16S) {  // cross terms, carrying the lo scale
+  if constexpr (PREC == kPrecF16S) {
 #pragma unroll
     for (int mo = 0; mo < 4; ++mo) accx[mo] = (f32x4)(0.0f);
-#pragma unroll
-    for (int st = 0; st < 2; ++st)
-#pragma unroll
-      for (int mo = 0; mo < 4; ++mo) {
-        const int f = (mo * 2 + st) * 64 + lane;
-        accx[mo] = mma(Al[f], bh[st], accx[mo]);
-        accx[mo] = mma(Ah[f], bl[st], accx[mo]);
-      }
   }
+  // one K-step at a time, so that at most 8 A fragments (32 VGPRs) are live
 #pragma unroll
-  for (int st = 0; st < 2; ++st)
+  for (int st = 0; st < 2; ++st) {
+    V8 ah[4], al[4];
 #pragma unroll
-    for (int mo = 0; mo < 4; ++mo) acc[mo] = mma(Ah[(mo * 2 + st) * 64 + lane], bh[st], acc[mo]);
+    for (int mo = 0; mo < 4; ++mo) {
+      const int f = (mo * 2 + st) * 64 + lane;
+      ah[mo] = Ah[f];
+      if constexpr (PREC == kPrecF16S) al[mo] = Al[f];
+    }
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo) {
+      if constexpr (PREC == kPrecF16S) {  // cross terms, carrying 2^kHalfExp
+        accx[mo] = mma(al[mo], bh[st], accx[mo]);
+        accx[mo] = mma(ah[mo], bl[st], accx[mo]);
+      }
+      acc[mo] = mma(ah[mo], bh[st], acc[mo]);
+    }
+    if (st == 0) __builtin_amdgcn_sched_barrier(0);
+  }
 }
 
 // One wave's walk of the op program over its 16 columns, and the ops.  The
@@ -251,9 +338,9 @@ __device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], f32x4 (&accx)[4], con
 // the arithmetic is the same code in both.
 template <int PREC, int WPG, bool PREF>
 struct Walk {
-  // live tiles of the FP16 GEMMs are kept scaled by 2^14 (their maximum in
-  // [2^13, 2^14)), the scale their GEMM operand needs
-  static constexpr int KE = (PREC == kPrecF16S || PREC == kPrecF16) ? kF16Exp : 0;
+  // live tiles of the FP16 GEMMs are kept with their maximum in
+  // [2^25, 2^26): 2^13 above the operand range [2^12, 2^13) gemm_half needs
+  static constexpr int KE = (PREC == kPrecF16S || PREC == kPrecF16) ? 2 * kHalfExp : 0;
   static constexpr int NT = 64 * WPG;        // threads
   static constexpr int PER = 4096 / NT / 4;  // float4 of the fragments per thread
   static constexpr int GC = 16 * WPG;        // columns per workgroup
@@ -269,12 +356,15 @@ struct Walk {
   int E = 0;                   // sum of the power-of-two rescaling exponents
   double LS = 0.0;             // sum of ln P[c][c] of the gathered leaves
   Gathered PF;                 // leaf gather issued ahead by the previous GEMM
+#ifdef PCSF_XLATE
+  f32x4 pfw[PER + 1];
+  bool pfw_ok = false;
+#endif
 
   __device__ __forceinline__ Walk(const MfmaParams& prm, float (*pb)[kTileFloats],
                                   const uint8_t* cs, int t, int l, int g, int cc)
       : p(prm), Pbuf(pb), codes_s(cs), tid(t), lane(l), grp(g), ccol(cc) {
     PF.marg = true;
-    PF.ln = 0.0;
   }
 
   // The transition set as an opaque scalar: each op recomputes its table
@@ -294,29 +384,33 @@ struct Walk {
     Gathered g1;
     gather(g1, p, codes_s, GC, oset(), a1, ccol, grp);
     Gathered& g0 = first_leaf(a0, use_pf);
-    s = leaf_tile(g0, LS);
+    s = leaf_tile(g0);
     k = __ballot(g0.marg && g1.marg);
-    mul_rescale<KE>(s, leaf_tile(g1, LS), E);
+    mul_rescale<KE>(s, leaf_tile(g1), E);
   }
   // s = message above the cherry (a0, a1), from table `tab`
   __device__ __forceinline__ void cherryt(Tile& s, uint64_t& k, int a0, int a1, int tab) {
     const int ca = codes_s[a0 * GC + ccol], cb = codes_s[a1 * GC + ccol];
     const bool ma = ca == PCSF_MARGINALIZE, mb = cb == PCSF_MARGINALIZE;
-    const size_t e = ((size_t)oset() * p.nctab + tab) * (kTabCodes * kTabCodes) +
-                     (ma ? 64 : ca) * kTabCodes + (mb ? 64 : cb);
-    const float* src = p.ctab + e * 64 + 4 * grp;
+    constexpr uint32_t kPairs = kTabCodes * kTabCodes;
+    const size_t t0 = (size_t)oset() * p.nctab * kPairs;  // this set's first table entry
+    const uint32_t e = (uint32_t)(ma ? 64 : ca) * kTabCodes + (mb ? 64 : cb);
+    const Rsrc rt = make_rsrc(p.ctab + t0 * 64, p.nctab * kPairs * 256u);
 #pragma unroll
-    for (int mbk = 0; mbk < 4; ++mbk) s.r[mbk] = *reinterpret_cast<const f32x4*>(src + 16 * mbk);
+    for (int mbk = 0; mbk < 4; ++mbk)
+      s.r[mbk] = bload4(rt, e * 256u + (uint32_t)grp * 16u + 64u * mbk, tab * kPairs * 256u);
     // (a table message always meets a sibling in LEAFMUL / MULPOP, whose
     // rescaling sets the tile scale, before it reaches a GEMM)
-    LS += p.clnt[e];
+    const Rsrc rl = make_rsrc(p.clnt + t0, p.nctab * kPairs * 8u);
+    LS += bload_f64(rl, e * 8u, tab * kPairs * 8u);
+    asm("" : "+v"(LS));  // as E in mul_rescale
     k = __ballot(ma && mb);
   }
   // s = s o leaf a0 (rescaled)
   __device__ __forceinline__ void leafmul(Tile& s, uint64_t& k, int a0, bool use_pf) {
     Gathered& g0 = first_leaf(a0, use_pf);
     k &= __ballot(g0.marg);
-    mul_rescale<KE>(s, leaf_tile(g0, LS), E);
+    mul_rescale<KE>(s, leaf_tile(g0), E);
   }
   // s = s o o (rescaled)
   __device__ __forceinline__ void mulpop(Tile& s, uint64_t& k, const Tile& o, uint64_t ko) {
@@ -327,18 +421,38 @@ struct Walk {
   // next_br (or the next task's first tile) and, if pf_leaf names one, the
   // next leaf gather
   __device__ __forceinline__ void gemm(Tile& s, uint64_t k, int next_br, int pf_leaf) {
+#ifdef PCSF_XLATE
+    if (pfw_ok) {
+      float* dst = Pbuf[kg & 1];
+#pragma unroll
+      for (int r = 0; r < PER; ++r) *reinterpret_cast<f32x4*>(&dst[(tid + NT * r) * 4]) = pfw[r];
+      if (tid < 16) *reinterpret_cast<f32x4*>(&dst[4096 + tid * 4]) = pfw[PER];
+    }
+    f32x4 (&pf)[PER + 1] = pfw;
+#endif
     __syncthreads();  // Pbuf[kg&1] complete; Pbuf[(kg&1)^1] no longer read
     const int st = oset();
     if (PREF && pf_leaf != kNoBranch) gather(PF, p, codes_s, GC, st, pf_leaf, ccol, grp);
-    const float* nsrc = nullptr;
-    if (next_br != kNoBranch) nsrc = p.Pfrag + ((size_t)st * p.nbr + next_br) * kTileFloats;
-    else if (has_next)
-      nsrc = p.Pfrag + ((size_t)next_set * p.nbr + p.first_gemm_br) * kTileFloats;
+    // the next tile: of this set (buffer loads from the set's tiles) or the
+    // next task's first (plain loads, once per task)
+    const bool nload = next_br != kNoBranch || has_next;
+#ifdef PCSF_XLATE
+    pfw_ok = nload;
+#else
     f32x4 pf[PER + 1];
-    if (nsrc) {
+#endif
+    if (next_br != kNoBranch) {
+      constexpr uint32_t kTileBytes = kTileFloats * 4;
+      const Rsrc r = make_rsrc(p.Pfrag + (size_t)st * p.nbr * kTileFloats, p.nbr * kTileBytes);
+      const uint32_t so = next_br * kTileBytes;
 #pragma unroll
-      for (int r = 0; r < PER; ++r)
-        pf[r] = *reinterpret_cast<const f32x4*>(nsrc + (tid + NT * r) * 4);
+      for (int q4 = 0; q4 < PER; ++q4) pf[q4] = bload4(r, (uint32_t)tid * 16u, so + 16u * NT * q4);
+      if (tid < 16) pf[PER] = bload4(r, (uint32_t)tid * 16u, so + 16384u);
+    } else if (has_next) {
+      const float* nsrc = p.Pfrag + ((size_t)next_set * p.nbr + p.first_gemm_br) * kTileFloats;
+#pragma unroll
+      for (int q4 = 0; q4 < PER; ++q4)
+        pf[q4] = *reinterpret_cast<const f32x4*>(nsrc + (tid + NT * q4) * 4);
       if (tid < 16) pf[PER] = *reinterpret_cast<const f32x4*>(nsrc + 4096 + tid * 4);
     }
     const float* Ab = Pbuf[kg & 1];
@@ -361,40 +475,52 @@ struct Walk {
         }
       }
     } else {
-      gemm_half<PREC>(acc, accx, s, Ab, lane);
+      gemm_half<PREC>(acc, accx, s, Ab, lane, k != 0, (k >> lane) & 1);
     }
-    // FP32: m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, the diagonal
-    // term in round-to-nearest VALU arithmetic; half precisions: the product
-    // is (P - I) alpha (carrying P's 2^14 for FP16, taken off in the FMA)
+    if constexpr (PREC == kPrecF32) {
+      // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, the diagonal term in
+      // round-to-nearest VALU arithmetic
 #pragma unroll
-    for (int mo = 0; mo < 4; ++mo) {
-      if constexpr (PREC == kPrecF32) {
+      for (int mo = 0; mo < 4; ++mo) {
         const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
 #pragma unroll
         for (int v = 0; v < 4; ++v)
           acc[mo][v] = s.r[mo][v] + fmaf(dd[v], s.r[mo][v], acc[mo][v]);
+      }
+      if (k == 0) {  // no lane of this wave holds a constant column (k is wave-uniform)
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo) s.r[mo] = acc[mo];
       } else {
+        const bool cst = (k >> lane) & 1;
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo) s.r[mo] = cst ? s.r[mo] : acc[mo];
+      }
+    } else {
+      // half precisions: m = s + (P - I) s, the identity term added in
+      // round-to-nearest VALU arithmetic (constant columns: + exactly 0)
+#ifdef PCSF_XSEL
+      const bool cst = k != 0 && ((k >> lane) & 1);
+#else
+      const bool cst = false;
+#endif
+#pragma unroll
+      for (int mo = 0; mo < 4; ++mo) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
+          float u;
           if constexpr (PREC == kPrecF16S)
-            acc[mo][v] =
-                fmaf(fmaf(accx[mo][v], kLoUnscale, acc[mo][v]), kF16Unscale, s.r[mo][v]);
-          else if constexpr (KE != 0)
-            acc[mo][v] = fmaf(acc[mo][v], kF16Unscale, s.r[mo][v]);
+            u = fmaf(accx[mo][v], kHalfUnscale, acc[mo][v]);
           else
-            acc[mo][v] = s.r[mo][v] + acc[mo][v];
+            u = acc[mo][v];
+          s.r[mo][v] += cst ? 0.0f : u;
         }
       }
     }
-    if (k == 0) {  // no lane of this wave holds a constant column (k is wave-uniform)
-#pragma unroll
-      for (int mo = 0; mo < 4; ++mo) s.r[mo] = acc[mo];
-    } else {
-      const bool cst = (k >> lane) & 1;
-#pragma unroll
-      for (int mo = 0; mo < 4; ++mo) s.r[mo] = cst ? s.r[mo] : acc[mo];
-    }
-    if (nsrc) {
+#ifdef PCSF_XLATE
+    if (false) {
+#else
+    if (nload) {
+#endif
       float* dst = Pbuf[(kg & 1) ^ 1];
 #pragma unroll
       for (int r = 0; r < PER; ++r) *reinterpret_cast<f32x4*>(&dst[(tid + NT * r) * 4]) = pf[r];
@@ -534,7 +660,7 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
     w.has_next = has_next;
     w.next_set = has_next ? nxt.set : 0;
     w.E = 0;
-    w.LS = 0.0;
+    w.LS = leaf_log_sum(p, codes_s, GC, cur.set, ccol);
     PROG::run(w, S, K);
     if (!has_next) break;
     cur = nxt;

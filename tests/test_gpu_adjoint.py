@@ -81,6 +81,50 @@ def test_backward_matches_oracle_branch_gradients(ctx, mammals, params_dir):
             assert np.allclose(gbr[r, q], g, rtol=1e-8, atol=1e-9), (r, q)
 
 
+@pytest.mark.parametrize("precision", [P.PCSF_FP32, P.PCSF_FP16S])
+def test_tangent_pass_matches_fp64_adjoint(ctx, mammals, precision):
+    """Backward of an MFMA-precision forward: the FP32 tangent pass of the
+    fused kernel against the FP64 outside pass (and the FP64 central
+    differences it is checked against above), empty and one-column regions,
+    gaps and N included."""
+    _say(f"tangent pass, precision {precision}")
+    m, codes, off, rho = mammals
+    b = P.Batch(m, codes, off)
+    b.forward(rho, P.PCSF_FP64, per_region=True)
+    g64, _ = b.backward(np.ones_like(rho))
+    b.forward(rho, precision, per_region=True)
+    g32, _ = b.backward(np.ones_like(rho))
+    assert np.all(np.abs(g32 - g64) <= 2e-4 * np.maximum(1.0, np.abs(g64))), (g32, g64)
+    assert np.all(g32[3] == 0.0)  # the empty region
+    # linear in the incoming gradient, as the FP64 pass
+    w = np.arange(1.0, 1.0 + rho.size).reshape(rho.shape)
+    g2, _ = b.backward(w)
+    assert np.allclose(g2, w * g32, rtol=1e-12, atol=1e-12)
+    # per-branch gradients always come from the FP64 outside pass
+    gb, gbr = b.backward(np.ones_like(rho), branches=True)
+    assert np.allclose(gb, g64, rtol=1e-12, atol=1e-12)
+
+
+def test_tangent_pass_58mammals(ctx):
+    """The tangent pass on the 58-leaf tree at several tree scales, against
+    the FP64 outside pass."""
+    _say("tangent pass, 58mammals")
+    m = P.PhyloCSFModel.from_paramset(ctx, "58mammals")
+    rng = np.random.default_rng(77)
+    R = 40
+    lengths = rng.integers(1, 300, R)
+    codes, off = m.simulate(lengths, rng.integers(0, 2, R), rng.uniform(0.3, 3.0, R),
+                            gap_p=0.2, n_p=0.01, seed=8)
+    rho = np.exp(rng.uniform(np.log(0.01), np.log(10.0), (R, 2)))
+    b = P.Batch(m, codes, off)
+    b.forward(rho, P.PCSF_FP64, per_region=True)
+    g64, _ = b.backward(np.ones_like(rho))
+    b.forward(rho, P.PCSF_FP32, per_region=True)
+    g32, _ = b.backward(np.ones_like(rho))
+    assert np.all(np.abs(g32 - g64) <= 2e-4 * np.maximum(1.0, np.abs(g64))), \
+        np.max(np.abs(g32 - g64) / np.maximum(1.0, np.abs(g64)))
+
+
 def test_autograd_function(ctx, mammals):
     _say("torch.autograd.Function")
     import torch
