@@ -176,21 +176,26 @@ int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
  * n_evals = likelihood evaluations per region.  Any output may be NULL. */
 #define PCSF_STRATEGY_FIXED 0
 #define PCSF_STRATEGY_MLE 1
-/* mle with the adjoint: root of d lpr / d ln rho on [0.01, 10] by Illinois
- * regula falsi (to 1e-7 in ln rho), boundary when the gradient keeps its sign;
- * finds the maximum the reference's 1%-tolerance Brent approximates. */
+/* mle with the gradient: root of h = d lpr / d ln rho on [0.01, 10] by
+ * Newton steps with h' from the last two evaluations, kept inside the
+ * bracket the signs of h leave and bisected when it stops halving, to 1e-6
+ * in ln rho (1e-4 with FP32 gradients); the boundary when h keeps its sign
+ * there.  Finds the maximum
+ * the reference's 1%-tolerance Brent approximates.  The gradient is the FP64
+ * outside pass at PCSF_FP64, the FP32 tangent pass otherwise. */
 #define PCSF_STRATEGY_MLE_GRAD 2
 int pcsf_score(pcsf_batch *b, int strategy, int precision, double *score, double *anc_comp,
                double *rho, double *lpr, double *elpr, int32_t *status, int32_t *n_evals);
 
 /* Autotuner of the fused kernel at one MFMA precision (PCSF_FP32 ..
  * PCSF_BF16): times every launch shape (waves per workgroup sharing one LDS
- * P tile x register cap x gather prefetch) on this batch at tree scale 1,
+ * P tile x register cap x gather prefetch x 16-column tiles per wave) on
+ * this batch at tree scale 1,
  * `reps` launches each, keeps the fastest on the batch's model for later
  * forwards at that precision and returns the per-shape milliseconds in
  * ms_out[PCSF_N_VARIANTS].  No counterpart in the reference (a tuning step
  * of this implementation). */
-#define PCSF_N_VARIANTS 6
+#define PCSF_N_VARIANTS 7
 int pcsf_autotune(pcsf_batch *b, int precision, int reps, double *ms_out, int *chosen);
 /* Select a launch shape by index (0..PCSF_N_VARIANTS-1) for one MFMA
  * precision without timing. */

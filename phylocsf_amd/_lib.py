@@ -22,6 +22,7 @@ PCSF_FP32 = 1
 PCSF_FP16S = 2
 PCSF_FP16 = 3
 PCSF_BF16 = 4
+PCSF_N_VARIANTS = 7  # launch shapes of the fused kernel (pcsf_autotune)
 
 EXPORTED = [
     "pcsf_ctx_create", "pcsf_ctx_destroy", "pcsf_last_error", "pcsf_ctx_set_timing",

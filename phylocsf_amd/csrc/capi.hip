@@ -217,7 +217,7 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
   }
   const int chunk_regions = b->rs.cap / nm;
   int variant = fp64 ? 0 : m->mfma_variant[precision];
-  if (16 * kMfmaVariants[variant].wpg > kAlignCols) variant = 1;
+  if (16 * kMfmaVariants[variant].wpg * kMfmaVariants[variant].g > kAlignCols) variant = 1;
   std::vector<uint8_t> act;
   b->t_pbuild = b->t_prune = b->t_reduce = 0.f;
   for (int32_t r0 = 0; r0 < b->nreg; r0 += chunk_regions) {
@@ -1087,13 +1087,29 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   pcsf_ctx* c = m->ctx;
   const size_t np = (size_t)b->nreg * m->nm;
   const double xlo = std::log(1e-2), xhi = std::log(10.0);
+  // ln rho tolerance: lpr is flat to second order at the maximum, so an
+  // error e in ln rho costs ~e^2/2 |d2 lpr / d (ln rho)^2| -- 1e-12 and 1e-8
+  // of it for the FP64 and the FP32-gradient tolerances.  FP32 gradients are
+  // noisy at the 1e-3 level near the root (a few thousand nats of lpr), so
+  // asking them for more than 1e-4 only buys bisection steps.
+  const double tol = (precision == PCSF_FP64) ? 1e-6 : 1e-4;
+  // Per pair: Newton steps on h = d lpr / d ln rho with h' from the last two
+  // evaluations (secant), inside the bracket [lo, hi] that the signs of h
+  // leave for the maximum; bisection when a step leaves the bracket or the
+  // bracket stops halving.
   struct G {
-    int phase = 0;  // 0: at 1, 1: probing end, 2: bracketed, 3: done
-    double x = 0, h = 0;
-    double a = 0, ha = 0, bb = 0, hb = 0;
-    int side = 0;  // Illinois bookkeeping
+    bool done = false, have_prev = false, seen_pos = false, seen_neg = false;
+    double x = 0, px = 0, ph = 0;
+    double lo = 0, hi = 0;
+    double w_check = 0;  // bracket width two steps ago
+    int since = 0;
   };
   std::vector<G> g(np);
+  for (G& s : g) {
+    s.lo = xlo;
+    s.hi = xhi;
+    s.w_check = xhi - xlo;
+  }
   st.assign(np, PairState());
   std::vector<double> rho(np, 1.0), L(np), D(np);
   std::vector<uint8_t> act(np, 1);
@@ -1105,10 +1121,10 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
     HIP_TRY(c, hipMemcpy(D.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost));
     return PCSF_OK;
   };
-  for (int it = 0; it < 200; ++it) {
+  for (int it = 0; it < 100; ++it) {
     bool any = false;
     for (size_t k = 0; k < np; ++k) {
-      act[k] = g[k].phase != 3;
+      act[k] = !g[k].done;
       rho[k] = std::exp(g[k].x);
       any = any || act[k];
     }
@@ -1122,57 +1138,57 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
       const double h = std::exp(s.x) * D[k];  // d lpr / d ln rho
       if (!std::isfinite(h) || !std::isfinite(L[k])) {
         st[k].status = PCSF_EFAIL;
-        s.phase = 3;
+        s.done = true;
         continue;
       }
-      if (s.phase == 0) {
-        s.a = s.x;
-        s.ha = h;
-        if (h == 0.0) {
-          s.phase = 3;
-        } else {
-          s.phase = 1;
-          s.x = (h > 0) ? xhi : xlo;
+      if (h == 0.0) {
+        s.done = true;
+        continue;
+      }
+      if (h > 0) {
+        s.seen_pos = true;
+        s.lo = std::max(s.lo, s.x);
+        if (s.x >= xhi) {  // increasing at the upper bound
+          s.done = true;
+          continue;
         }
-      } else if (s.phase == 1) {
-        if ((s.ha > 0 && h >= 0) || (s.ha < 0 && h <= 0)) {
-          s.phase = 3;  // maximum on the boundary
-        } else {
-          s.bb = s.x;
-          s.hb = h;
-          s.phase = 2;
-          s.side = 0;
-        }
-      } else if (s.phase == 2) {
-        if (h == 0.0) {
-          s.phase = 3;
-        } else if ((h > 0) == (s.ha > 0)) {
-          s.a = s.x;
-          s.ha = h;
-          if (s.side == -1) s.hb *= 0.5;
-          s.side = -1;
-        } else {
-          s.bb = s.x;
-          s.hb = h;
-          if (s.side == 1) s.ha *= 0.5;
-          s.side = 1;
+      } else {
+        s.seen_neg = true;
+        s.hi = std::min(s.hi, s.x);
+        if (s.x <= xlo) {
+          s.done = true;
+          continue;
         }
       }
-      if (s.phase == 2) {
-        if (std::fabs(s.bb - s.a) <= 1e-7) {
-          s.x = 0.5 * (s.a + s.bb);
-          s.phase = 3;
-        } else {
-          double xn = (s.a * s.hb - s.bb * s.ha) / (s.hb - s.ha);
-          const double lo = std::min(s.a, s.bb), hi = std::max(s.a, s.bb);
-          if (!(xn > lo && xn < hi)) xn = 0.5 * (s.a + s.bb);
-          if (std::fabs(xn - s.x) <= 1e-9) {
-            s.phase = 3;  // converged on the same point
-          } else {
-            s.x = xn;
-          }
-        }
+      if (s.hi - s.lo <= tol) {
+        s.x = 0.5 * (s.lo + s.hi);
+        s.done = true;
+        continue;
       }
+      double xn;
+      if (!s.have_prev) {
+        xn = s.x + (h > 0 ? 1.0 : -1.0);  // first move: a factor e in rho
+      } else {
+        const double hp = (h - s.ph) / (s.x - s.px);
+        xn = (hp < 0) ? s.x - h / hp : s.x + (h > 0 ? 2.0 : -2.0) * std::fabs(s.x - s.px);
+      }
+      const bool bracketed = s.seen_pos && s.seen_neg;
+      if (bracketed && ++s.since >= 2) {  // the bracket must halve every two steps
+        if (s.hi - s.lo > 0.5 * s.w_check) xn = 0.5 * (s.lo + s.hi);
+        s.w_check = s.hi - s.lo;
+        s.since = 0;
+      }
+      if (!(xn > s.lo && xn < s.hi))
+        xn = bracketed ? 0.5 * (s.lo + s.hi) : std::min(std::max(xn, s.lo), s.hi);
+      if (std::fabs(xn - s.x) <= tol) {
+        s.x = xn;
+        s.done = true;
+        continue;
+      }
+      s.px = s.x;
+      s.ph = h;
+      s.have_prev = true;
+      s.x = xn;
     }
   }
   // lpr and elpr at the estimates, in the requested precision

@@ -174,9 +174,10 @@ struct MfmaVariant {
   int wpg;        // waves per workgroup (16 columns each) sharing one LDS P tile
   int minw;       // minimum waves per SIMD the compiler must allow (register cap)
   bool prefetch;  // GEMMs issue the next leaf gather ahead (+18 VGPRs)
+  int g;          // 16-column tiles per wave, walked side by side
   const char* what;
 };
-constexpr int kNumMfmaVariants = 6;
+constexpr int kNumMfmaVariants = 7;
 extern const MfmaVariant kMfmaVariants[kNumMfmaVariants];
 hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int precision,
                              int variant, hipStream_t s);

@@ -11,12 +11,13 @@ template hipError_t launch_variant<kPrecF32>(const MfmaParams&, int, int, int, h
 using namespace pcsf_mfma;
 
 const MfmaVariant kMfmaVariants[kNumMfmaVariants] = {
-    {8, 1, true, "8 waves/WG, gather prefetch"},
-    {4, 2, true, "4 waves/WG, >=2 waves/SIMD, gather prefetch"},
-    {4, 3, false, "4 waves/WG, >=3 waves/SIMD"},
-    {2, 2, true, "2 waves/WG, >=2 waves/SIMD, gather prefetch"},
-    {4, 2, false, "4 waves/WG, >=2 waves/SIMD"},
-    {2, 3, false, "2 waves/WG, >=3 waves/SIMD"},
+    {8, 1, true, 1, "8 waves/WG, gather prefetch"},
+    {4, 2, true, 1, "4 waves/WG, >=2 waves/SIMD, gather prefetch"},
+    {4, 3, false, 1, "4 waves/WG, >=3 waves/SIMD"},
+    {2, 2, true, 1, "2 waves/WG, >=2 waves/SIMD, gather prefetch"},
+    {4, 2, false, 1, "4 waves/WG, >=2 waves/SIMD"},
+    {2, 3, false, 1, "2 waves/WG, >=3 waves/SIMD"},
+    {2, 1, false, 2, "2 waves/WG, 2 column tiles per wave"},
 };
 
 namespace {
@@ -47,11 +48,7 @@ MfmaParams base_params(const pcsf_model* m, const PSets& sets) {
   prm.nl = m->nl;
   prm.nm = m->nm;
   prm.nbr = m->nbr;
-#ifdef PCSF_XNOTAB
-  const bool tables = false;
-#else
   const bool tables = sets.d_ctab != nullptr && !m->cherries.empty();
-#endif
   prm.first_gemm_br = tables ? m->first_gemm_br_t : m->first_gemm_br;
   prm.prog = tables ? m->d_prog_t : m->d_prog;
   prm.ctab = tables ? sets.d_ctab : nullptr;
@@ -93,11 +90,11 @@ hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int preci
                              int variant, hipStream_t s) {
   if (variant < 0 || variant >= kNumMfmaVariants) return hipErrorInvalidValue;
   if (m->fixed.fmt != frag_format(precision)) return hipErrorInvalidValue;
-  const int wpg = kMfmaVariants[variant].wpg;
+  const int gc = 16 * kMfmaVariants[variant].wpg * kMfmaVariants[variant].g;
   MfmaParams prm = base_params(m, m->fixed);
   prm.codes = b->d_codes;
   prm.ld = b->ld;
-  prm.ngroups = b->ld / (16 * wpg);
+  prm.ngroups = b->ld / gc;
   prm.logz = b->d_logz;
   prm.colelpr = b->d_colelpr;
   if (prm.ngroups == 0) return hipSuccess;
@@ -112,7 +109,7 @@ hipError_t launch_prune_mfma_regions(const pcsf_model* m, const pcsf_batch* b, i
                                      int variant, const RegionChunk& ch, hipStream_t s) {
   if (variant < 0 || variant >= kNumMfmaVariants) return hipErrorInvalidValue;
   if (ch.sets->fmt != frag_format(precision)) return hipErrorInvalidValue;
-  const int gc = 16 * kMfmaVariants[variant].wpg;
+  const int gc = 16 * kMfmaVariants[variant].wpg * kMfmaVariants[variant].g;
   if (gc > kAlignCols) return hipErrorInvalidValue;  // a group must not span regions
   MfmaParams prm = base_params(m, *ch.sets);
   prm.codes = b->al.d_codes;

@@ -156,24 +156,12 @@ typedef __amdgpu_buffer_rsrc_t Rsrc;
 __device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
 }
-#ifndef PCSF_XGLOBAL
 __device__ __forceinline__ f32x4 bload4(Rsrc r, uint32_t voff, uint32_t soff) {
   return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 __device__ __forceinline__ double bload_f64(Rsrc r, uint32_t voff, uint32_t soff) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
-#else
-struct XR { const char* b; };
-#define make_rsrc(base, bytes) XR{reinterpret_cast<const char*>(base)}
-#define Rsrc XR
-__device__ __forceinline__ f32x4 bload4(XR r, uint32_t voff, uint32_t soff) {
-  return *reinterpret_cast<const f32x4*>(r.b + soff + voff);
-}
-__device__ __forceinline__ double bload_f64(XR r, uint32_t voff, uint32_t soff) {
-  return *reinterpret_cast<const double*>(r.b + soff + voff);
-}
-#endif
 
 struct Gathered {
   Tile t;
@@ -227,17 +215,14 @@ __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
     for (int v = 0; v < 4; ++v) mx = max(mx, __float_as_int(s.r[mb][v]));
   }
   // lanes l, l^16, l^32, l^48 hold the column
-#ifdef PCSF_XPERM
+  // (two cross-row swaps, each returning the value and its partner's: VALU
+  // work, no LDS round trip)
   {
     const auto a = __builtin_amdgcn_permlane16_swap(mx, mx, false, false);
     mx = max((int)a[0], (int)a[1]);
     const auto b = __builtin_amdgcn_permlane32_swap(mx, mx, false, false);
     mx = max((int)b[0], (int)b[1]);
   }
-#else
-  mx = max(mx, __shfl_xor(mx, 16));
-  mx = max(mx, __shfl_xor(mx, 32));
-#endif
   const int e = (mx > 0) ? __builtin_amdgcn_frexp_expf(__int_as_float(mx)) : 0;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb)
@@ -250,66 +235,80 @@ __device__ __forceinline__ void mul_rescale(Tile& s, const Tile& o, int& E) {
   asm("" : "+v"(E));
 }
 
+// G tiles of 16 columns each, walked by one wave side by side: the ops of
+// the groups are independent, so the compiler interleaves their dependency
+// chains, and one A fragment read from LDS feeds the MFMAs of every group.
+template <int G>
+struct TileG {
+  Tile g[G];
+};
+template <int G>
+struct MaskG {  // per group: lanes whose column is a constant vector
+  uint64_t m[G];
+};
+
 // acc = (P - I) s in half precision for the P - I tile staged in LDS (Ab)
-// and the 64x16 tile s; F16S leaves the cross terms hi*lo + lo*hi, which
-// carry 2^kHalfExp, in accx.  The tile is kept at 2^kHalfExp above its GEMM
-// operand (FP16: operand maximum in [2^12, 2^13)), so the product comes out
-// at the tile's scale, and hi = RN(s 2^-13) and the residual RN(s - 2^13 hi)
-// are one mixed-precision FMA each.  Both accumulators start from zero (the
-// matrix cores' FP32 accumulation rounds toward zero, which would bias s if
-// it were the initial value).  Columns flagged cst (wave-uniform any_cst: at
-// least one) take a zero operand, so their update is exactly zero.  Tile
-// layout of the halves
-// (pbuild.hip): hi fragments, then lo fragments, 512 x 8 values each, in the
-// A-operand order of v_mfma_f32_16x16x32_{f16,bf16} with K permuted so that
-// K-step st of a lane in group g takes rows 16*(2*st + j/4) + 4*g + j%4 of
-// its column: exactly what the accumulator layout leaves in registers
-// s.r[2*st + j/4][j%4].
-template <int PREC>
-__device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], f32x4 (&accx)[4], const Tile& s,
-                                          const float* Ab, int lane, bool any_cst, bool cst) {
+// and each group's 64x16 tile; F16S leaves the cross terms hi*lo + lo*hi,
+// which carry 2^kHalfExp, in accx.  The tile is kept at 2^kHalfExp above
+// its GEMM operand (FP16: operand maximum in [2^12, 2^13)), so the product
+// comes out at the tile's scale, and hi = RN(s 2^-13) and the residual
+// RN(s - 2^13 hi) are one mixed-precision FMA each.  Both accumulators
+// start from zero (the matrix cores' FP32 accumulation rounds toward zero,
+// which would bias s if it were the initial value).  Columns of constant
+// tiles (mask k) take a zero operand, so their update is exactly zero.
+// Tile layout of the halves (pbuild.hip): hi fragments, then lo fragments,
+// 512 x 8 values each, in the A-operand order of
+// v_mfma_f32_16x16x32_{f16,bf16} with K permuted so that K-step st of a lane
+// in group g takes rows 16*(2*st + j/4) + 4*g + j%4 of its column: exactly
+// what the accumulator layout leaves in registers s.r[2*st + j/4][j%4].
+template <int PREC, int G>
+__device__ __forceinline__ void gemm_half(f32x4 (&acc)[G][4], f32x4 (&accx)[G][4],
+                                          const TileG<G>& s, const float* Ab, int lane,
+                                          const MaskG<G>& k) {
   constexpr bool kBF = PREC == kPrecBF16;
   using V8 = typename std::conditional<kBF, bf16x8, f16x8>::type;
   using E = typename std::conditional<kBF, __bf16, _Float16>::type;
   constexpr float kDown = kBF ? 1.0f : 1.0f / (float)(1 << kHalfExp);
-  V8 bh[2], bl[2];
+  V8 bh[G][2], bl[G][2];
 #pragma unroll
-  for (int st = 0; st < 2; ++st) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) bh[st][j] = (E)(s.r[2 * st + (j >> 2)][j & 3] * kDown);
-    if constexpr (PREC == kPrecF16S) {
-      // the residual is taken against the rounded hi in its register (the
-      // compiler would otherwise recompute hi inside each residual FMA)
-      asm("" : "+v"(bh[st]));
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        bl[st][j] = (E)fmaf((float)bh[st][j], -(float)(1 << kHalfExp),
-                            s.r[2 * st + (j >> 2)][j & 3]);
-    }
-  }
-#ifdef PCSF_XSEL
-  if (false) {
-#else
-  if (any_cst) {
-#endif
+  for (int c = 0; c < G; ++c) {
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
-      bh[st] = cst ? (V8)(E)0 : bh[st];
-      if constexpr (PREC == kPrecF16S) bl[st] = cst ? (V8)(E)0 : bl[st];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        bh[c][st][j] = (E)(s.g[c].r[2 * st + (j >> 2)][j & 3] * kDown);
+      if constexpr (PREC == kPrecF16S) {
+        // the residual is taken against the rounded hi in its register (the
+        // compiler would otherwise recompute hi inside each residual FMA)
+        asm("" : "+v"(bh[c][st]));
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          bl[c][st][j] = (E)fmaf((float)bh[c][st][j], -(float)(1 << kHalfExp),
+                                 s.g[c].r[2 * st + (j >> 2)][j & 3]);
+      }
+    }
+    if (k.m[c] != 0) {  // wave-uniform
+      const bool cst = (k.m[c] >> lane) & 1;
+#pragma unroll
+      for (int st = 0; st < 2; ++st) {
+        bh[c][st] = cst ? (V8)(E)0 : bh[c][st];
+        if constexpr (PREC == kPrecF16S) bl[c][st] = cst ? (V8)(E)0 : bl[c][st];
+      }
     }
   }
   const V8* Ah = reinterpret_cast<const V8*>(Ab);
   const V8* Al = Ah + 512;
-#pragma unroll
-  for (int mo = 0; mo < 4; ++mo) acc[mo] = (f32x4)(0.0f);
-  auto mma = [](const V8& a, const V8& b, const f32x4& c) -> f32x4 {
-    if constexpr (kBF) return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-    else return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  auto mma = [](const V8& a, const V8& b, const f32x4& cc) -> f32x4 {
+    if constexpr (kBF) return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, cc, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, cc, 0, 0, 0);
   };
-  if constexpr (PREC == kPrecF16S) {
 #pragma unroll
-    for (int mo = 0; mo < 4; ++mo) accx[mo] = (f32x4)(0.0f);
-  }
+  for (int c = 0; c < G; ++c)
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo) {
+      acc[c][mo] = (f32x4)(0.0f);
+      if constexpr (PREC == kPrecF16S) accx[c][mo] = (f32x4)(0.0f);
+    }
   // one K-step at a time, so that at most 8 A fragments (32 VGPRs) are live
 #pragma unroll
   for (int st = 0; st < 2; ++st) {
@@ -321,50 +320,60 @@ __device__ __forceinline__ void gemm_half(f32x4 (&acc)[4], f32x4 (&accx)[4], con
       if constexpr (PREC == kPrecF16S) al[mo] = Al[f];
     }
 #pragma unroll
-    for (int mo = 0; mo < 4; ++mo) {
-      if constexpr (PREC == kPrecF16S) {  // cross terms, carrying 2^kHalfExp
-        accx[mo] = mma(al[mo], bh[st], accx[mo]);
-        accx[mo] = mma(ah[mo], bl[st], accx[mo]);
+    for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+      for (int c = 0; c < G; ++c) {
+        if constexpr (PREC == kPrecF16S) {  // cross terms, carrying 2^kHalfExp
+          accx[c][mo] = mma(al[mo], bh[c][st], accx[c][mo]);
+          accx[c][mo] = mma(ah[mo], bl[c][st], accx[c][mo]);
+        }
+        acc[c][mo] = mma(ah[mo], bh[c][st], acc[c][mo]);
       }
-      acc[mo] = mma(ah[mo], bh[st], acc[mo]);
-    }
     if (st == 0) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-// One wave's walk of the op program over its 16 columns, and the ops.  The
-// interpreter (Interp) and the tree-specialised straight-line programs
+// One wave's walk of the op program over its G x 16 columns, and the ops.
+// The interpreter (Interp) and the tree-specialised straight-line programs
 // (generated per shipped tree at build time) both drive these members, so
 // the arithmetic is the same code in both.
-template <int PREC, int WPG, bool PREF>
+template <int PREC, int WPG, bool PREF, int G>
 struct Walk {
   // live tiles of the FP16 GEMMs are kept with their maximum in
   // [2^25, 2^26): 2^13 above the operand range [2^12, 2^13) gemm_half needs
   static constexpr int KE = (PREC == kPrecF16S || PREC == kPrecF16) ? 2 * kHalfExp : 0;
   static constexpr int NT = 64 * WPG;        // threads
   static constexpr int PER = 4096 / NT / 4;  // float4 of the fragments per thread
-  static constexpr int GC = 16 * WPG;        // columns per workgroup
+  static constexpr int GC = 16 * WPG * G;    // columns per workgroup
+  using T = TileG<G>;
+  using M = MaskG<G>;
   const MfmaParams& p;
   float (*Pbuf)[kTileFloats];  // LDS: double-buffered P tile
+  typedef __attribute__((address_space(3))) float L3;
+  L3 (*Pbuf3)[kTileFloats] = nullptr;  // the same, as LDS addresses
   const uint8_t* codes_s;      // LDS: the group's leaf codes, nl x GC
-  int tid, lane, grp, ccol;
+  int tid, lane, grp;
+  int ccol[G];                 // this lane's columns within the workgroup's GC
   int q = 0, set = 0;          // model and transition set of the current task
-  int64_t col = 0;             // this lane's column
+  int64_t col[G];              // this lane's columns
   bool has_next = false;       // a next task exists (its first tile is prefetched)
   int next_set = 0;
   int kg = 0;                  // GEMMs done (selects the LDS buffer)
-  int E = 0;                   // sum of the power-of-two rescaling exponents
-  double LS = 0.0;             // sum of ln P[c][c] of the gathered leaves
-  Gathered PF;                 // leaf gather issued ahead by the previous GEMM
-#ifdef PCSF_XLATE
-  f32x4 pfw[PER + 1];
-  bool pfw_ok = false;
-#endif
+  int E[G];                    // sums of the power-of-two rescaling exponents
+  double LS[G];                // sums of ln P[c][c] of the gathered leaves
+  Gathered PF[G];              // leaf gathers issued ahead by the previous GEMM
 
   __device__ __forceinline__ Walk(const MfmaParams& prm, float (*pb)[kTileFloats],
-                                  const uint8_t* cs, int t, int l, int g, int cc)
-      : p(prm), Pbuf(pb), codes_s(cs), tid(t), lane(l), grp(g), ccol(cc) {
-    PF.marg = true;
+                                  const uint8_t* cs, int t, int l, int g, int cc0)
+      : p(prm), Pbuf(pb), codes_s(cs), tid(t), lane(l), grp(g) {
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      ccol[c] = cc0 + 16 * c;
+      col[c] = 0;
+      E[c] = 0;
+      LS[c] = 0.0;
+      PF[c].marg = true;
+    }
   }
 
   // The transition set as an opaque scalar: each op recomputes its table
@@ -375,91 +384,111 @@ struct Walk {
     asm volatile("" : "+s"(s));
     return s;
   }
-  __device__ __forceinline__ Gathered& first_leaf(int leaf, bool use_pf) {
-    if (!(PREF && use_pf)) gather(PF, p, codes_s, GC, oset(), leaf, ccol, grp);
-    return PF;
+  __device__ __forceinline__ Gathered& first_leaf(int c, int leaf, bool use_pf) {
+    if (!(PREF && use_pf)) gather(PF[c], p, codes_s, GC, oset(), leaf, ccol[c], grp);
+    return PF[c];
   }
   // s = leaf a0 o leaf a1 (rescaled)
-  __device__ __forceinline__ void cherry(Tile& s, uint64_t& k, int a0, int a1, bool use_pf) {
-    Gathered g1;
-    gather(g1, p, codes_s, GC, oset(), a1, ccol, grp);
-    Gathered& g0 = first_leaf(a0, use_pf);
-    s = leaf_tile(g0);
-    k = __ballot(g0.marg && g1.marg);
-    mul_rescale<KE>(s, leaf_tile(g1), E);
+  __device__ __forceinline__ void cherry(T& s, M& k, int a0, int a1, bool use_pf) {
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      Gathered g1;
+      gather(g1, p, codes_s, GC, oset(), a1, ccol[c], grp);
+      Gathered& g0 = first_leaf(c, a0, use_pf);
+      s.g[c] = leaf_tile(g0);
+      k.m[c] = __ballot(g0.marg && g1.marg);
+      mul_rescale<KE>(s.g[c], leaf_tile(g1), E[c]);
+    }
   }
   // s = message above the cherry (a0, a1), from table `tab`
-  __device__ __forceinline__ void cherryt(Tile& s, uint64_t& k, int a0, int a1, int tab) {
-    const int ca = codes_s[a0 * GC + ccol], cb = codes_s[a1 * GC + ccol];
-    const bool ma = ca == PCSF_MARGINALIZE, mb = cb == PCSF_MARGINALIZE;
+  __device__ __forceinline__ void cherryt(T& s, M& k, int a0, int a1, int tab) {
     constexpr uint32_t kPairs = kTabCodes * kTabCodes;
     const size_t t0 = (size_t)oset() * p.nctab * kPairs;  // this set's first table entry
-    const uint32_t e = (uint32_t)(ma ? 64 : ca) * kTabCodes + (mb ? 64 : cb);
     const Rsrc rt = make_rsrc(p.ctab + t0 * 64, p.nctab * kPairs * 256u);
-#pragma unroll
-    for (int mbk = 0; mbk < 4; ++mbk)
-      s.r[mbk] = bload4(rt, e * 256u + (uint32_t)grp * 16u + 64u * mbk, tab * kPairs * 256u);
-    // (a table message always meets a sibling in LEAFMUL / MULPOP, whose
-    // rescaling sets the tile scale, before it reaches a GEMM)
     const Rsrc rl = make_rsrc(p.clnt + t0, p.nctab * kPairs * 8u);
-    LS += bload_f64(rl, e * 8u, tab * kPairs * 8u);
-    asm("" : "+v"(LS));  // as E in mul_rescale
-    k = __ballot(ma && mb);
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      const int ca = codes_s[a0 * GC + ccol[c]], cb = codes_s[a1 * GC + ccol[c]];
+      const bool ma = ca == PCSF_MARGINALIZE, mb = cb == PCSF_MARGINALIZE;
+      const uint32_t e = (uint32_t)(ma ? 64 : ca) * kTabCodes + (mb ? 64 : cb);
+#pragma unroll
+      for (int mbk = 0; mbk < 4; ++mbk)
+        s.g[c].r[mbk] =
+            bload4(rt, e * 256u + (uint32_t)grp * 16u + 64u * mbk, tab * kPairs * 256u);
+      // (a table message always meets a sibling in LEAFMUL / MULPOP, whose
+      // rescaling sets the tile scale, before it reaches a GEMM)
+      LS[c] += bload_f64(rl, e * 8u, tab * kPairs * 8u);
+      asm("" : "+v"(LS[c]));  // as E in mul_rescale
+      k.m[c] = __ballot(ma && mb);
+    }
   }
   // s = s o leaf a0 (rescaled)
-  __device__ __forceinline__ void leafmul(Tile& s, uint64_t& k, int a0, bool use_pf) {
-    Gathered& g0 = first_leaf(a0, use_pf);
-    k &= __ballot(g0.marg);
-    mul_rescale<KE>(s, leaf_tile(g0), E);
+  __device__ __forceinline__ void leafmul(T& s, M& k, int a0, bool use_pf) {
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      Gathered& g0 = first_leaf(c, a0, use_pf);
+      k.m[c] &= __ballot(g0.marg);
+      mul_rescale<KE>(s.g[c], leaf_tile(g0), E[c]);
+    }
   }
   // s = s o o (rescaled)
-  __device__ __forceinline__ void mulpop(Tile& s, uint64_t& k, const Tile& o, uint64_t ko) {
-    k &= ko;
-    mul_rescale<KE>(s, o, E);
+  __device__ __forceinline__ void mulpop(T& s, M& k, const T& o, const M& ko) {
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      k.m[c] &= ko.m[c];
+      mul_rescale<KE>(s.g[c], o.g[c], E[c]);
+    }
   }
   // s = P s for the P tile staged in LDS; prefetches the tile of branch
   // next_br (or the next task's first tile) and, if pf_leaf names one, the
   // next leaf gather
-  __device__ __forceinline__ void gemm(Tile& s, uint64_t k, int next_br, int pf_leaf) {
-#ifdef PCSF_XLATE
-    if (pfw_ok) {
-      float* dst = Pbuf[kg & 1];
-#pragma unroll
-      for (int r = 0; r < PER; ++r) *reinterpret_cast<f32x4*>(&dst[(tid + NT * r) * 4]) = pfw[r];
-      if (tid < 16) *reinterpret_cast<f32x4*>(&dst[4096 + tid * 4]) = pfw[PER];
-    }
-    f32x4 (&pf)[PER + 1] = pfw;
-#endif
+  __device__ __forceinline__ void gemm(T& s, const M& k, int next_br, int pf_leaf) {
     __syncthreads();  // Pbuf[kg&1] complete; Pbuf[(kg&1)^1] no longer read
     const int st = oset();
-    if (PREF && pf_leaf != kNoBranch) gather(PF, p, codes_s, GC, st, pf_leaf, ccol, grp);
-    // the next tile: of this set (buffer loads from the set's tiles) or the
-    // next task's first (plain loads, once per task)
-    const bool nload = next_br != kNoBranch || has_next;
-#ifdef PCSF_XLATE
-    pfw_ok = nload;
-#else
-    f32x4 pf[PER + 1];
-#endif
-    if (next_br != kNoBranch) {
-      constexpr uint32_t kTileBytes = kTileFloats * 4;
-      const Rsrc r = make_rsrc(p.Pfrag + (size_t)st * p.nbr * kTileFloats, p.nbr * kTileBytes);
-      const uint32_t so = next_br * kTileBytes;
+    if (PREF && pf_leaf != kNoBranch) {
 #pragma unroll
-      for (int q4 = 0; q4 < PER; ++q4) pf[q4] = bload4(r, (uint32_t)tid * 16u, so + 16u * NT * q4);
-      if (tid < 16) pf[PER] = bload4(r, (uint32_t)tid * 16u, so + 16384u);
-    } else if (has_next) {
-      const float* nsrc = p.Pfrag + ((size_t)next_set * p.nbr + p.first_gemm_br) * kTileFloats;
+      for (int c = 0; c < G; ++c) gather(PF[c], p, codes_s, GC, st, pf_leaf, ccol[c], grp);
+    }
+    // the next tile, of this set (buffer loads from the set's tiles) or the
+    // next task's first (plain loads, once per task), goes straight from
+    // memory into the idle LDS buffer: no registers, no LDS stores, and the
+    // barrier that opens the next GEMM waits for it.  Each instruction
+    // writes 16 bytes per lane at the wave's LDS base + 16 * lane, the
+    // layout the tile has in memory.
+    {
+      L3* dst = Pbuf3[(kg & 1) ^ 1];
+      typedef __attribute__((address_space(3))) void* LdsPtr;
+      if (next_br != kNoBranch) {
+        constexpr uint32_t kTileBytes = kTileFloats * 4;
+        const Rsrc r = make_rsrc(p.Pfrag + (size_t)st * p.nbr * kTileFloats, p.nbr * kTileBytes);
+        const uint32_t so = next_br * kTileBytes;
 #pragma unroll
-      for (int q4 = 0; q4 < PER; ++q4)
-        pf[q4] = *reinterpret_cast<const f32x4*>(nsrc + (tid + NT * q4) * 4);
-      if (tid < 16) pf[PER] = *reinterpret_cast<const f32x4*>(nsrc + 4096 + tid * 4);
+        for (int q4 = 0; q4 < PER; ++q4)
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(
+              r, (LdsPtr)(dst + ((tid & ~63) + NT * q4) * 4), 16, (uint32_t)tid * 16u,
+              so + 16u * NT * q4, 0, 0);
+        if (tid < 16)
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (LdsPtr)(dst + 4096), 16,
+                                                   (uint32_t)tid * 16u, so + 16384u, 0, 0);
+      } else if (has_next) {
+        typedef const __attribute__((address_space(1))) void* GPtr;
+        const float* nsrc = p.Pfrag + ((size_t)next_set * p.nbr + p.first_gemm_br) * kTileFloats;
+#pragma unroll
+        for (int q4 = 0; q4 < PER; ++q4)
+          __builtin_amdgcn_global_load_lds((GPtr)(nsrc + (tid + NT * q4) * 4),
+                                           (LdsPtr)(dst + ((tid & ~63) + NT * q4) * 4), 16, 0, 0);
+        if (tid < 16)
+          __builtin_amdgcn_global_load_lds((GPtr)(nsrc + 4096 + tid * 4), (LdsPtr)(dst + 4096),
+                                           16, 0, 0);
+      }
     }
     const float* Ab = Pbuf[kg & 1];
-    f32x4 acc[4], accx[4];
+    f32x4 acc[G][4], accx[G][4];
     if constexpr (PREC == kPrecF32) {
 #pragma unroll
-      for (int mo = 0; mo < 4; ++mo) acc[mo] = (f32x4)(0.0f);
+      for (int c = 0; c < G; ++c)
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo) acc[c][mo] = (f32x4)(0.0f);
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
         f32x4 A[4];
@@ -467,88 +496,79 @@ struct Walk {
         for (int mo = 0; mo < 4; ++mo)
           A[mo] = *reinterpret_cast<const f32x4*>(Ab + ((mo * 4 + mb) * 64 + lane) * 4);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const float bv = s.r[mb][v];
+        for (int v = 0; v < 4; ++v)
 #pragma unroll
-          for (int mo = 0; mo < 4; ++mo)
-            acc[mo] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mo][v], bv, acc[mo], 0, 0, 0);
-        }
+          for (int c = 0; c < G; ++c) {
+            const float bv = s.g[c].r[mb][v];
+#pragma unroll
+            for (int mo = 0; mo < 4; ++mo)
+              acc[c][mo] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mo][v], bv, acc[c][mo], 0, 0, 0);
+          }
       }
-    } else {
-      gemm_half<PREC>(acc, accx, s, Ab, lane, k != 0, (k >> lane) & 1);
-    }
-    if constexpr (PREC == kPrecF32) {
       // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, the diagonal term in
       // round-to-nearest VALU arithmetic
 #pragma unroll
       for (int mo = 0; mo < 4; ++mo) {
         const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          acc[mo][v] = s.r[mo][v] + fmaf(dd[v], s.r[mo][v], acc[mo][v]);
+        for (int c = 0; c < G; ++c)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            acc[c][mo][v] = s.g[c].r[mo][v] + fmaf(dd[v], s.g[c].r[mo][v], acc[c][mo][v]);
       }
-      if (k == 0) {  // no lane of this wave holds a constant column (k is wave-uniform)
 #pragma unroll
-        for (int mo = 0; mo < 4; ++mo) s.r[mo] = acc[mo];
-      } else {
-        const bool cst = (k >> lane) & 1;
+      for (int c = 0; c < G; ++c) {
+        if (k.m[c] == 0) {  // no lane holds a constant column (wave-uniform)
 #pragma unroll
-        for (int mo = 0; mo < 4; ++mo) s.r[mo] = cst ? s.r[mo] : acc[mo];
-      }
-    } else {
-      // half precisions: m = s + (P - I) s, the identity term added in
-      // round-to-nearest VALU arithmetic (constant columns: + exactly 0)
-#ifdef PCSF_XSEL
-      const bool cst = k != 0 && ((k >> lane) & 1);
-#else
-      const bool cst = false;
-#endif
+          for (int mo = 0; mo < 4; ++mo) s.g[c].r[mo] = acc[c][mo];
+        } else {
+          const bool cst = (k.m[c] >> lane) & 1;
 #pragma unroll
-      for (int mo = 0; mo < 4; ++mo) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          float u;
-          if constexpr (PREC == kPrecF16S)
-            u = fmaf(accx[mo][v], kHalfUnscale, acc[mo][v]);
-          else
-            u = acc[mo][v];
-          s.r[mo][v] += cst ? 0.0f : u;
+          for (int mo = 0; mo < 4; ++mo) s.g[c].r[mo] = cst ? s.g[c].r[mo] : acc[c][mo];
         }
       }
-    }
-#ifdef PCSF_XLATE
-    if (false) {
-#else
-    if (nload) {
-#endif
-      float* dst = Pbuf[(kg & 1) ^ 1];
+    } else {
+      gemm_half<PREC, G>(acc, accx, s, Ab, lane, k);
+      // m = s + (P - I) s, the identity term added in round-to-nearest VALU
+      // arithmetic (constant columns: + exactly 0)
 #pragma unroll
-      for (int r = 0; r < PER; ++r) *reinterpret_cast<f32x4*>(&dst[(tid + NT * r) * 4]) = pf[r];
-      if (tid < 16) *reinterpret_cast<f32x4*>(&dst[4096 + tid * 4]) = pf[PER];
+      for (int c = 0; c < G; ++c)
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            if constexpr (PREC == kPrecF16S)
+              s.g[c].r[mo][v] += fmaf(accx[c][mo][v], kHalfUnscale, acc[c][mo][v]);
+            else
+              s.g[c].r[mo][v] += acc[c][mo][v];
+          }
     }
     ++kg;
   }
-  // ln z and elpr of this lane's column from the root tile
-  __device__ __forceinline__ void root(const Tile& s) {
+  // ln z and elpr of this lane's columns from the root tiles
+  __device__ __forceinline__ void root(const T& s) {
     const double* piq = p.pi + q * 64;
     const double* lnq = p.lnpi + q * 64;
-    double z = 0.0, el = 0.0;
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+    for (int c = 0; c < G; ++c) {
+      double z = 0.0, el = 0.0;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = 16 * mb + 4 * grp + v;
-        const double x = (double)s.r[mb][v] * piq[row];
-        z += x;
-        el = fma(x, lnq[row], el);
+      for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * mb + 4 * grp + v;
+          const double x = (double)s.g[c].r[mb][v] * piq[row];
+          z += x;
+          el = fma(x, lnq[row], el);
+        }
+      z += __shfl_xor(z, 16);
+      el += __shfl_xor(el, 16);
+      z += __shfl_xor(z, 32);
+      el += __shfl_xor(el, 32);
+      if (grp == 0) {
+        p.logz[(size_t)q * p.ld + col[c]] = log(z) + (double)E[c] * 0.69314718055994530942 + LS[c];
+        p.colelpr[(size_t)q * p.ld + col[c]] = (z == 0.0) ? 0.0 : el / z;
       }
-    z += __shfl_xor(z, 16);
-    el += __shfl_xor(el, 16);
-    z += __shfl_xor(z, 32);
-    el += __shfl_xor(el, 32);
-    if (grp == 0) {
-      p.logz[(size_t)q * p.ld + col] = log(z) + (double)E * 0.69314718055994530942 + LS;
-      p.colelpr[(size_t)q * p.ld + col] = (z == 0.0) ? 0.0 : el / z;
     }
   }
 };
@@ -557,8 +577,8 @@ struct Walk {
 // and dispatches on (opcode, slot).
 template <int NS>
 struct Interp {
-  template <class W>
-  __device__ __forceinline__ static void run(W& w, Tile (&S)[NS], uint64_t (&K)[NS]) {
+  template <class W, class T, class M>
+  __device__ __forceinline__ static void run(W& w, T (&S)[NS], M (&K)[NS]) {
     const int32_t* prog = w.p.prog;
     for (int pc = 0;; ++pc) {
       const int w0 = __builtin_amdgcn_readfirstlane(prog[4 * pc]);
@@ -602,9 +622,9 @@ struct Interp {
   }
 };
 
-template <int PREC, int NS, int WPG, int MINW, bool PREF, class PROG>
+template <int PREC, int NS, int WPG, int MINW, bool PREF, class PROG, int G>
 __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p) {
-  using W = Walk<PREC, WPG, PREF>;
+  using W = Walk<PREC, WPG, PREF, G>;
   constexpr int NT = W::NT, PER = W::PER, GC = W::GC;
   static_assert(NT >= 16, "the diagonal (16 float4) is copied by the first 16 threads");
   __shared__ __attribute__((aligned(16))) float Pbuf[2][kTileFloats];
@@ -612,14 +632,16 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int grp = lane >> 4, lcol = lane & 15;
 
-  Tile S[NS];
-  uint64_t K[NS];  // lanes whose tile column is a constant vector
+  TileG<G> S[NS];
+  MaskG<G> K[NS];
 #pragma unroll
-  for (int d = 0; d < NS; ++d) {
-    K[d] = 0;
+  for (int d = 0; d < NS; ++d)
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) S[d].r[mb] = (f32x4)(0.0f);
-  }
+    for (int c = 0; c < G; ++c) {
+      K[d].m[c] = 0;
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) S[d].g[c].r[mb] = (f32x4)(0.0f);
+    }
 
   Task cur;
   int64_t jc = 0;
@@ -635,8 +657,9 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
           *reinterpret_cast<const f32x4*>(src + 4096 + tid * 4);
   }
 
-  const int ccol = wave * 16 + lcol;  // column within the workgroup's group
-  W w(p, Pbuf, codes_s, tid, lane, grp, ccol);
+  // wave w holds columns [16 G w, 16 G (w + 1)) of the workgroup's group
+  W w(p, Pbuf, codes_s, tid, lane, grp, wave * 16 * G + lcol);
+  w.Pbuf3 = (typename W::L3 (*)[kTileFloats])Pbuf;
   for (;;) {
     Task nxt;
     int64_t jn = jc + 1;
@@ -656,11 +679,14 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
     __syncthreads();
     w.q = cur.q;
     w.set = cur.set;
-    w.col = gabs * GC + ccol;
     w.has_next = has_next;
     w.next_set = has_next ? nxt.set : 0;
-    w.E = 0;
-    w.LS = leaf_log_sum(p, codes_s, GC, cur.set, ccol);
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      w.col[c] = gabs * GC + w.ccol[c];
+      w.E[c] = 0;
+      w.LS[c] = leaf_log_sum(p, codes_s, GC, cur.set, w.ccol[c]);
+    }
     PROG::run(w, S, K);
     if (!has_next) break;
     cur = nxt;
@@ -668,10 +694,10 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
   }
 }
 
-template <int PREC, int NS, int WPG, int MINW, bool PREF, class PROG = Interp<NS>>
+template <int PREC, int NS, int WPG, int MINW, bool PREF, class PROG = Interp<NS>, int G = 1>
 hipError_t launch_ns(const MfmaParams& prm, int grid_per_cu, int num_cus, hipStream_t s) {
-  auto kern = prune_mfma_kernel<PREC, NS, WPG, MINW, PREF, PROG>;
-  const size_t dyn = ((size_t)prm.nl * 16 * WPG + 15) / 16 * 16;  // staged leaf codes
+  auto kern = prune_mfma_kernel<PREC, NS, WPG, MINW, PREF, PROG, G>;
+  const size_t dyn = ((size_t)prm.nl * 16 * WPG * G + 15) / 16 * 16;  // staged leaf codes
   if (grid_per_cu <= 0) {
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WPG, dyn);
@@ -683,15 +709,15 @@ hipError_t launch_ns(const MfmaParams& prm, int grid_per_cu, int num_cus, hipStr
   return hipGetLastError();
 }
 
-template <int PREC, int WPG, int MINW, bool PREF>
+template <int PREC, int WPG, int MINW, bool PREF, int G = 1>
 hipError_t launch_v(const MfmaParams& prm, int slots, int gpc, int cus, hipStream_t s) {
   switch (slots) {
     case 1:
-    case 2: return launch_ns<PREC, 2, WPG, MINW, PREF>(prm, gpc, cus, s);
-    case 3: return launch_ns<PREC, 3, WPG, MINW, PREF>(prm, gpc, cus, s);
-    case 4: return launch_ns<PREC, 4, WPG, MINW, PREF>(prm, gpc, cus, s);
-    case 5: return launch_ns<PREC, 5, WPG, MINW, PREF>(prm, gpc, cus, s);
-    case 6: return launch_ns<PREC, 6, WPG, MINW, PREF>(prm, gpc, cus, s);
+    case 2: return launch_ns<PREC, 2, WPG, MINW, PREF, Interp<2>, G>(prm, gpc, cus, s);
+    case 3: return launch_ns<PREC, 3, WPG, MINW, PREF, Interp<3>, G>(prm, gpc, cus, s);
+    case 4: return launch_ns<PREC, 4, WPG, MINW, PREF, Interp<4>, G>(prm, gpc, cus, s);
+    case 5: return launch_ns<PREC, 5, WPG, MINW, PREF, Interp<5>, G>(prm, gpc, cus, s);
+    case 6: return launch_ns<PREC, 6, WPG, MINW, PREF, Interp<6>, G>(prm, gpc, cus, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -708,6 +734,7 @@ hipError_t launch_variant(const MfmaParams& prm, int variant, int slots, int cus
     case 3: return launch_v<PREC, 2, 2, true>(prm, slots, 0, cus, s);
     case 4: return launch_v<PREC, 4, 2, false>(prm, slots, 0, cus, s);
     case 5: return launch_v<PREC, 2, 3, false>(prm, slots, 0, cus, s);
+    case 6: return launch_v<PREC, 2, 1, false, 2>(prm, slots, 0, cus, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -718,4 +745,3 @@ extern template hipError_t launch_variant<kPrecF16>(const MfmaParams&, int, int,
 extern template hipError_t launch_variant<kPrecBF16>(const MfmaParams&, int, int, int, hipStream_t);
 
 }  // namespace pcsf_mfma
-

@@ -152,8 +152,35 @@ def test_autotune_each_precision(ctx):
     ref = P.score_fixed(b, P.PCSF_FP64)
     for prec in (P.PCSF_FP32, P.PCSF_FP16S):
         chosen, ms = b.autotune(reps=1, precision=prec)
-        assert 0 <= chosen < 6 and np.all(ms > 0)
+        assert 0 <= chosen < P.PCSF_N_VARIANTS and np.all(ms > 0)
         dmax, _, _ = _errors(P.score_fixed(b, prec), ref)
         assert dmax <= TOL_DB
     with pytest.raises(P.InvalidArgument):
         b.autotune(reps=1, precision=P.PCSF_FP64)
+
+
+@pytest.mark.parametrize("specialized", [True, False])
+def test_every_launch_shape(ctx, specialized):
+    """Every launch shape of the fused kernel (waves per workgroup, register
+    cap, gather prefetch, 16-column tiles per wave), tree-specialised and
+    interpreted, fixed and per-region scales, against FP64."""
+    _say("every launch shape, specialized=%s" % specialized)
+    m = W.model_for("C2", ctx)
+    m.specialized(specialized)
+    wl = W.make("C2", m, n_regions=40)
+    b = P.Batch(m, wl.codes, wl.offsets)
+    ref = P.score_fixed(b, P.PCSF_FP64)
+    rng = np.random.default_rng(4)
+    rho = rng.uniform(0.3, 3.0, (b.n_regions, 2))
+    ll64, _ = b.forward(rho, P.PCSF_FP64, per_region=True)
+    try:
+        for prec in (P.PCSF_FP32, P.PCSF_FP16S):
+            for v in range(P.PCSF_N_VARIANTS):
+                m.set_variant(v, prec)
+                dmax, _, _ = _errors(P.score_fixed(b, prec), ref)
+                assert dmax <= TOL_DB, (prec, v, dmax)
+                ll, _ = b.forward(rho, prec, per_region=True)
+                rel = np.max(np.abs(ll - ll64) / np.abs(ll64))
+                assert rel <= 1e-6, (prec, v, rel)
+    finally:
+        m.specialized(True)
