@@ -135,13 +135,16 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="threads of the CPU baseline (the GPU host's share is 16 cores)")
     ap.add_argument("--regions", type=int, default=None, help="override region count")
-    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fp16s",
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default=None,
                     help="GEMM precision of the batched MFMA path (fp16s and fp32 meet "
-                         "the 1e-3 dB bound; fp16 and bf16 do not)")
+                         "the 1e-3 dB bound; fp16 and bf16 do not); default fp16s, and "
+                         "fp32 for C4 (its configuration names FP32)")
     ap.add_argument("--workload", choices=["C2", "C3", "C4", "C5"], default="C2",
                     help="C2 (default) is the headline measurement; C3/C4/C5 print their own "
                          "line for the other scoring configurations")
     args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "fp32" if args.workload == "C4" else "fp16s"
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (regions shard "
               "across GPUs by design only); run with --gpus 1", file=sys.stderr)
@@ -324,7 +327,8 @@ def other_workload(args):
         el_s = time.perf_counter() - t1
         score = P.DB * (ll[:, 0] - ll[:, 1])
         best_frame = score.reshape(-1, 6).argmax(axis=1)
-        _line("codon-columns/s, 6-frame fixed scoring (FP32 MFMA)", batch.n_cols * steps / el_s,
+        _line("codon-columns/s, 6-frame fixed scoring (%s MFMA)" % args.precision.upper(),
+              batch.n_cols * steps / el_s,
               "codon-columns/s", steps, args.warmup, 1000 * el_s / steps, dtype,
               {"workload": "120mammals, %d regions U{20..600}, 30/70 coding/noncoding, gap runs "
                            "(mean 5 codons) covering 0.25, 6 reading frames, fixed strategy (seed 4)"

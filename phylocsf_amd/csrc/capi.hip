@@ -1141,7 +1141,9 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
         s.done = true;
         continue;
       }
-      if (h == 0.0) {
+      // h = 0, or (FP32 gradients) below their rounding noise, ~1e-8 |lpr|:
+      // the root is as close as these gradients can place it
+      if (h == 0.0 || (precision != PCSF_FP64 && std::fabs(h) <= 1e-7 * std::fabs(L[k]))) {
         s.done = true;
         continue;
       }

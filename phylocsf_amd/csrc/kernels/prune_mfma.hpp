@@ -56,6 +56,7 @@
 //    Live partials stay FP32 in registers in every precision.  The FP32 path
 //    keeps the diagonal of P in an FP32 VALU term; the half-precision tiles
 //    hold P - I, so their product is the whole (small) update of alpha.
+#pragma once
 #include <type_traits>
 
 #include "../internal.hpp"

@@ -8,6 +8,7 @@
 // back to the interpreter.
 #pragma once
 #include "prune_mfma.hpp"
+#include "prune_tangent.hpp"
 
 namespace pcsf_mfma {
 
@@ -15,13 +16,15 @@ typedef hipError_t (*TreeLaunch)(const MfmaParams&, int precision, int variant, 
                                  hipStream_t);
 
 // One generated program: the tree's name, whether it reads cherry tables,
-// its op words (to match a model's program) and its launcher.
+// its op words (to match a model's program), its launcher and (programs
+// without tables) the launcher of its tangent pass.
 struct TreeProgram {
   const char* name;
   int tables;
   const int32_t* ops;  // nullptr: the tree has no specialised program
   int n_words;
   TreeLaunch launch;
+  TreeLaunch tangent;
 };
 
 // Every generated program (host function of the generated registry).

@@ -123,6 +123,15 @@ def test_tangent_pass_58mammals(ctx):
     g32, _ = b.backward(np.ones_like(rho))
     assert np.all(np.abs(g32 - g64) <= 2e-4 * np.maximum(1.0, np.abs(g64))), \
         np.max(np.abs(g32 - g64) / np.maximum(1.0, np.abs(g64)))
+    # the tree-specialised tangent kernel against the interpreted one
+    assert m.specialized()["plain"]
+    m.specialized(False)
+    try:
+        b.forward(rho, P.PCSF_FP32, per_region=True)
+        gi, _ = b.backward(np.ones_like(rho))
+    finally:
+        m.specialized(True)
+    assert np.allclose(gi, g32, rtol=1e-9, atol=1e-9), np.max(np.abs(gi - g32))
 
 
 def test_autograd_function(ctx, mammals):
