@@ -20,7 +20,7 @@ constexpr int kLeafRows = 65;  // leaf messages per branch: codons 0..63, 64 = `
 // (region, model) of a chunk of regions).
 // Formats of the P tile (PSets::d_Pfrag, kTileFloats floats per branch):
 //   kFragF32   off-diagonal FP32 in v_mfma_f32_16x16x4f32 A order + diagonal
-//   kFragF16   x = (P - I) x 2^13 as FP16 hi = RN(x), then the FP16 residual
+//   kFragF16   x = offdiag(P) x 2^13 as FP16 hi = RN(x), then the FP16 residual
 //              lo = RN((x - hi) x 2^13), in v_mfma_f32_16x16x32_f16 A order
 //              (4096 halves each) + diagonal
 //   kFragBF16  the same in BF16, unscaled, residual unscaled

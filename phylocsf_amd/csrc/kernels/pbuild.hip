@@ -16,8 +16,9 @@
 //         row 64 is the all-ones message of `Marginalize (lnPcc 0).
 //   Pfrag tile of kTileFloats floats in format fmt, in the A-operand order of
 //         the MFMA the kernel uses: FP32 off-diagonal of P (diagonal entries
-//         zero) for v_mfma_f32_16x16x4f32, or hi and lo halves of P - I (FP16
-//         scaled by 2^kHalfExp, or BF16) for v_mfma_f32_16x16x32_{f16,bf16};
+//         zero) for v_mfma_f32_16x16x4f32, or hi and lo halves of the same
+//         off-diagonal (FP16 scaled by 2^kHalfExp, or BF16) for
+//         v_mfma_f32_16x16x32_{f16,bf16};
 //         followed by the 64 FP32 values P[a][a] - 1 (read by the FP32 path).
 #include "../internal.hpp"
 
@@ -123,16 +124,17 @@ __global__ __launch_bounds__(256) void pbuild_kernel(
     //   e = ((mo*2 + st)*64 + lane)*8 + j
     //   row = 16*mo + lane%16, col = 16*(2*st + j/4) + 4*(lane/16) + j%4
     // hi = x rounded to nearest, lo = the residual x - hi rounded to nearest;
-    // FP16: x = (P - I) 2^13 and lo carries a further 2^13 (a normal number
+    // FP16: x = offdiag(P) 2^13 and lo carries a further 2^13 (a normal number
     // wherever hi is: the matrix cores read FP16 subnormals as zero)
     const double sc = (fmt == kFragF16) ? (double)(1 << kHalfExp) : 1.0;
     for (int e = tid; e < 4096; e += 256) {
       const int j = e & 7, lane = (e >> 3) & 63, st = (e >> 9) & 1, mo = e >> 10;
       const int row = 16 * mo + (lane & 15);
       const int col = 16 * (2 * st + (j >> 2)) + 4 * (lane >> 4) + (j & 3);
-      // the half-precision tiles hold P - I (the diagonal term rides in the
-      // product, which starts from zero and stays small next to alpha)
-      const double x = ((row == col) ? Vs[row * 64 + col] - 1.0 : Vs[row * 64 + col]) * sc;
+      // the off-diagonal of P, as in the FP32 tile: the products then sum
+      // nonnegative terms, and the diagonal P[a][a] - 1 (float 4096 on)
+      // joins in round-to-nearest VALU arithmetic
+      const double x = (row == col) ? 0.0 : Vs[row * 64 + col] * sc;
       if (fmt == kFragF16) {
         _Float16* h = reinterpret_cast<_Float16*>(Pfrag + fbase);
         const _Float16 hi = (_Float16)x;

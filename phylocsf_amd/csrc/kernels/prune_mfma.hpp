@@ -20,20 +20,22 @@
 //    matching permuted order (pbuild.hip) and read with ds_read_b128.
 //  * FP32 MFMA accumulation rounds with a bias toward zero (measured on
 //    MI355X: about -2e-6 per column and model when the accumulator carries
-//    alpha).  So the MFMA only accumulates the off-diagonal mass of P from a
-//    zero accumulator, and m = alpha + fma(P_aa - 1, alpha, offdiag) is
+//    alpha).  So in every precision the MFMA only accumulates the
+//    off-diagonal mass of P from a zero accumulator (nonnegative terms, no
+//    cancellation), and m = alpha + fma(P_aa - 1, alpha, offdiag) is
 //    finished in round-to-nearest VALU arithmetic.
 //  * Leaf children are gathers of a column of P normalised by its diagonal
 //    entry (so the dominant entry is exactly 1; ln P[c][c] is summed per
-//    column in FP64), or 1 for `Marginalize; cherries need no GEMM.  A lane mask per live tile marks columns whose
-//    partial is a constant vector (a fully marginalised subtree); P c1 = c1
-//    exactly, so those columns skip the GEMM update instead of picking up
-//    the rounding of the FP32 row sums of P - I.  After every product each
+//    column in FP64), or 1 for `Marginalize; cherries need no GEMM.  A lane
+//    mask per live tile marks columns whose partial is a constant vector (a
+//    fully marginalised subtree); P c1 = c1 exactly, so those columns skip
+//    the GEMM update instead of picking up the rounding of the row sums of
+//    P.  After every product each
 //    column is rescaled by the power of two of its maximum (exact) and the
 //    exponents are summed per column; the root dot and the logarithms are
 //    FP64.
-//  * P tiles are double-buffered in LDS: while a GEMM runs, every thread
-//    prefetches its share of the next GEMM's tile from L2.  The workgroup's
+//  * P tiles are double-buffered in LDS: while a GEMM runs, the next GEMM's
+//    tile is loaded from memory straight into the idle buffer.  The workgroup's
 //    leaf codes are staged in LDS once per task, and each GEMM also issues
 //    the loads of the next leaf gather (named by the host's op program), so
 //    gathers do not wait on a dependent chain of global loads.
@@ -41,8 +43,8 @@
 //    L2 holds one model's matrices.
 //  * GEMM precision (template PREC): FP32 on v_mfma_f32_16x16x4f32; or P and
 //    the partial enter v_mfma_f32_16x16x32_{f16,bf16} in half precision with
-//    FP32 accumulation.  FP16 tiles hold (P - I) 2^13 (entries below 2^13 in
-//    magnitude); the live tiles of the FP16 kernels are kept with their
+//    FP32 accumulation.  FP16 tiles hold offdiag(P) 2^13 (entries below 2^13);
+//    the live tiles of the FP16 kernels are kept with their
 //    column maximum in [2^25, 2^26) by the rescaling itself and enter the
 //    GEMM as RN(s 2^-13), maximum in [2^12, 2^13): nothing overflows, values
 //    down to 2^-26 of the maximum stay normal, and the product comes out at
@@ -53,9 +55,7 @@
 //    unscaled lo), and forms hi*hi in one accumulator and hi*lo + lo*hi in
 //    another: about 23 significant bits per product at three FP16 MFMAs,
 //    against sixteen FP32 MFMAs for the same 64x64x16 product.
-//    Live partials stay FP32 in registers in every precision.  The FP32 path
-//    keeps the diagonal of P in an FP32 VALU term; the half-precision tiles
-//    hold P - I, so their product is the whole (small) update of alpha.
+//    Live partials stay FP32 in registers in every precision.
 #pragma once
 #include <type_traits>
 
@@ -248,24 +248,21 @@ struct MaskG {  // per group: lanes whose column is a constant vector
   uint64_t m[G];
 };
 
-// acc = (P - I) s in half precision for the P - I tile staged in LDS (Ab)
-// and each group's 64x16 tile; F16S leaves the cross terms hi*lo + lo*hi,
-// which carry 2^kHalfExp, in accx.  The tile is kept at 2^kHalfExp above
-// its GEMM operand (FP16: operand maximum in [2^12, 2^13)), so the product
-// comes out at the tile's scale, and hi = RN(s 2^-13) and the residual
-// RN(s - 2^13 hi) are one mixed-precision FMA each.  Both accumulators
-// start from zero (the matrix cores' FP32 accumulation rounds toward zero,
-// which would bias s if it were the initial value).  Columns of constant
-// tiles (mask k) take a zero operand, so their update is exactly zero.
-// Tile layout of the halves (pbuild.hip): hi fragments, then lo fragments,
+// acc = offdiag(P) s in half precision for the tile staged in LDS (Ab) and
+// each group's 64x16 tile; F16S leaves the cross terms hi*lo + lo*hi, which
+// carry 2^kHalfExp, in accx.  The tile is kept at 2^kHalfExp above its GEMM
+// operand (FP16: operand maximum in [2^12, 2^13)), so the product comes out
+// at the tile's scale, and hi = RN(s 2^-13) and the residual RN(s - 2^13 hi)
+// are one mixed-precision FMA each.  Both accumulators start from zero (the
+// matrix cores' FP32 accumulation rounds toward zero, which would bias s if
+// it were the initial value).  Tile layout of the halves (pbuild.hip): hi fragments, then lo fragments,
 // 512 x 8 values each, in the A-operand order of
 // v_mfma_f32_16x16x32_{f16,bf16} with K permuted so that K-step st of a lane
 // in group g takes rows 16*(2*st + j/4) + 4*g + j%4 of its column: exactly
 // what the accumulator layout leaves in registers s.r[2*st + j/4][j%4].
 template <int PREC, int G>
 __device__ __forceinline__ void gemm_half(f32x4 (&acc)[G][4], f32x4 (&accx)[G][4],
-                                          const TileG<G>& s, const float* Ab, int lane,
-                                          const MaskG<G>& k) {
+                                          const TileG<G>& s, const float* Ab, int lane) {
   constexpr bool kBF = PREC == kPrecBF16;
   using V8 = typename std::conditional<kBF, bf16x8, f16x8>::type;
   using E = typename std::conditional<kBF, __bf16, _Float16>::type;
@@ -286,14 +283,6 @@ __device__ __forceinline__ void gemm_half(f32x4 (&acc)[G][4], f32x4 (&accx)[G][4
         for (int j = 0; j < 8; ++j)
           bl[c][st][j] = (E)fmaf((float)bh[c][st][j], -(float)(1 << kHalfExp),
                                  s.g[c].r[2 * st + (j >> 2)][j & 3]);
-      }
-    }
-    if (k.m[c] != 0) {  // wave-uniform
-      const bool cst = (k.m[c] >> lane) & 1;
-#pragma unroll
-      for (int st = 0; st < 2; ++st) {
-        bh[c][st] = cst ? (V8)(E)0 : bh[c][st];
-        if constexpr (PREC == kPrecF16S) bl[c][st] = cst ? (V8)(E)0 : bl[c][st];
       }
     }
   }
@@ -529,20 +518,33 @@ struct Walk {
         }
       }
     } else {
-      gemm_half<PREC, G>(acc, accx, s, Ab, lane, k);
-      // m = s + (P - I) s, the identity term added in round-to-nearest VALU
-      // arithmetic (constant columns: + exactly 0)
+      gemm_half<PREC, G>(acc, accx, s, Ab, lane);
+      // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, as in FP32: the
+      // half-precision products sum nonnegative terms only, the diagonal
+      // and the identity are round-to-nearest VALU arithmetic
 #pragma unroll
-      for (int c = 0; c < G; ++c)
+      for (int mo = 0; mo < 4; ++mo) {
+        const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
 #pragma unroll
-        for (int mo = 0; mo < 4; ++mo)
+        for (int c = 0; c < G; ++c)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            if constexpr (PREC == kPrecF16S)
-              s.g[c].r[mo][v] += fmaf(accx[c][mo][v], kHalfUnscale, acc[c][mo][v]);
-            else
-              s.g[c].r[mo][v] += acc[c][mo][v];
+            float u = acc[c][mo][v];
+            if constexpr (PREC == kPrecF16S) u = fmaf(accx[c][mo][v], kHalfUnscale, u);
+            acc[c][mo][v] = s.g[c].r[mo][v] + fmaf(dd[v], s.g[c].r[mo][v], u);
           }
+      }
+#pragma unroll
+      for (int c = 0; c < G; ++c) {
+        if (k.m[c] == 0) {  // no lane holds a constant column (wave-uniform)
+#pragma unroll
+          for (int mo = 0; mo < 4; ++mo) s.g[c].r[mo] = acc[c][mo];
+        } else {
+          const bool cst = (k.m[c] >> lane) & 1;
+#pragma unroll
+          for (int mo = 0; mo < 4; ++mo) s.g[c].r[mo] = cst ? s.g[c].r[mo] : acc[c][mo];
+        }
+      }
     }
     ++kg;
   }
