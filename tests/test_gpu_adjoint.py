@@ -105,10 +105,11 @@ def test_tangent_pass_matches_fp64_adjoint(ctx, mammals, precision):
     assert np.allclose(gb, g64, rtol=1e-12, atol=1e-12)
 
 
-def test_tangent_pass_58mammals(ctx):
-    """The tangent pass on the 58-leaf tree at several tree scales, against
-    the FP64 outside pass."""
-    _say("tangent pass, 58mammals")
+@pytest.mark.parametrize("precision", [P.PCSF_FP32, P.PCSF_FP16S])
+def test_tangent_pass_58mammals(ctx, precision):
+    """The FP32 tangent pass after an FP32 or split-FP16 forward on the
+    58-leaf tree at several tree scales, against the FP64 outside pass."""
+    _say(f"tangent pass, 58mammals, precision {precision}")
     m = P.PhyloCSFModel.from_paramset(ctx, "58mammals")
     rng = np.random.default_rng(77)
     R = 40
@@ -119,7 +120,7 @@ def test_tangent_pass_58mammals(ctx):
     b = P.Batch(m, codes, off)
     b.forward(rho, P.PCSF_FP64, per_region=True)
     g64, _ = b.backward(np.ones_like(rho))
-    b.forward(rho, P.PCSF_FP32, per_region=True)
+    b.forward(rho, precision, per_region=True)
     g32, _ = b.backward(np.ones_like(rho))
     assert np.all(np.abs(g32 - g64) <= 2e-4 * np.maximum(1.0, np.abs(g64))), \
         np.max(np.abs(g32 - g64) / np.maximum(1.0, np.abs(g64)))
@@ -127,7 +128,7 @@ def test_tangent_pass_58mammals(ctx):
     assert m.specialized()["plain"]
     m.specialized(False)
     try:
-        b.forward(rho, P.PCSF_FP32, per_region=True)
+        b.forward(rho, precision, per_region=True)
         gi, _ = b.backward(np.ones_like(rho))
     finally:
         m.specialized(True)
