@@ -111,6 +111,31 @@ def test_other_trees_fixed(ctx, params_dir):
         _check(P.score_fixed(b, P.PCSF_FP32), ref, TOL_DB_FP32, TOL_REL_FP32, name + "/FP32")
 
 
+ALL_PARAMSETS = ["100vertebrates", "120mammals", "12flies", "20flies", "21mosquitoes",
+                 "23flies", "26worms", "29mammals", "30mammals_mm39", "35vertebrates_mm39",
+                 "49birds", "53birds", "58mammals", "7yeast"]
+
+
+@pytest.mark.parametrize("name", ALL_PARAMSETS)
+def test_every_parameter_set_fixed(ctx, params_dir, name):
+    """Every shipped parameter set (trees of 7 to 120 leaves, 20flies' zero-length
+    branch, the interpreter for trees without a generated program): FP64, FP32
+    and split-FP16 fixed scores against the oracle on a small simulated sample
+    with gaps and N."""
+    _say("parameter set %s" % name)
+    m = P.PhyloCSFModel.from_paramset(ctx, name)
+    rng = np.random.default_rng(len(name))
+    lengths = np.array([0, 1, 17, 64, 65, 130])
+    codes, off = m.simulate(lengths, rng.integers(0, 2, len(lengths)),
+                            rng.uniform(0.3, 3.0, len(lengths)), gap_p=0.15, n_p=0.02,
+                            seed=len(name))
+    ref = _oracle_scores(_oracle_model(params_dir, name), codes, off)
+    b = P.Batch(m, codes, off)
+    _check(P.score_fixed(b, P.PCSF_FP64), ref, TOL_DB_FP64, TOL_REL_FP64, name + "/FP64")
+    _check(P.score_fixed(b, P.PCSF_FP32), ref, TOL_DB_FP32, TOL_REL_FP32, name + "/FP32")
+    _check(P.score_fixed(b, P.PCSF_FP16S), ref, TOL_DB_FP32, TOL_REL_FP32, name + "/FP16S")
+
+
 def test_examples_fixed(ctx, params_dir, examples_dir):
     """The reference's example alignments, fixed strategy, frame 1."""
     from oracle import cli_oracle as CO
