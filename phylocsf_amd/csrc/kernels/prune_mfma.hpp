@@ -7,9 +7,9 @@
 // PhyloCSFModel.lpr_leaves (src/PhyloCSFModel.ml:67-82), for every packed
 // codon column and every model, organised for CDNA4:
 //
-//  * A wave owns a tile of 16 columns; the WPG waves of a workgroup walk the
-//    same tree in lock step, so one P matrix staged in LDS feeds 16*WPG
-//    columns.  Partial likelihoods never leave registers: the host compiles
+//  * A wave owns G tiles of 16 columns (G = 1 but for one launch shape);
+//    the WPG waves of a workgroup walk the same tree in lock step, so one P
+//    matrix staged in LDS feeds 16*G*WPG columns.  Partial likelihoods never leave registers: the host compiles
 //    the tree into an op program (post-order, subtree needing more live
 //    tiles first) that keeps at most NS 64x16 tiles (16 VGPRs each) live.
 //  * An internal-child message m = P alpha is a 64x64 by 64x16 GEMM on
@@ -30,22 +30,21 @@
 //    mask per live tile marks columns whose partial is a constant vector (a
 //    fully marginalised subtree); P c1 = c1 exactly, so those columns skip
 //    the GEMM update instead of picking up the rounding of the row sums of
-//    P.  After every product each
-//    column is rescaled by the power of two of its maximum (exact) and the
-//    exponents are summed per column; the root dot and the logarithms are
-//    FP64.
+//    P.  After every product each column is rescaled by the power of two of
+//    its maximum (exact) and the exponents are summed per column; the root
+//    dot and the logarithms are FP64.
 //  * P tiles are double-buffered in LDS: while a GEMM runs, the next GEMM's
-//    tile is loaded from memory straight into the idle buffer.  The workgroup's
-//    leaf codes are staged in LDS once per task, and each GEMM also issues
-//    the loads of the next leaf gather (named by the host's op program), so
-//    gathers do not wait on a dependent chain of global loads.
+//    tile is loaded from memory straight into the idle buffer.  The
+//    workgroup's leaf codes are staged in LDS once per task, and (launch
+//    shapes with gather prefetch) each GEMM also issues the loads of the next
+//    leaf gather named by the host's op program.
 //  * Workgroups are mapped to models by XCD (blockIdx % 8), so each XCD's
 //    L2 holds one model's matrices.
 //  * GEMM precision (template PREC): FP32 on v_mfma_f32_16x16x4f32; or P and
 //    the partial enter v_mfma_f32_16x16x32_{f16,bf16} in half precision with
-//    FP32 accumulation.  FP16 tiles hold offdiag(P) 2^13 (entries below 2^13);
-//    the live tiles of the FP16 kernels are kept with their
-//    column maximum in [2^25, 2^26) by the rescaling itself and enter the
+//    FP32 accumulation.  FP16 tiles hold offdiag(P) 2^13 (entries below
+//    2^13); the live tiles of the FP16 kernels are kept with their column
+//    maximum in [2^25, 2^26) by the rescaling itself and enter the
 //    GEMM as RN(s 2^-13), maximum in [2^12, 2^13): nothing overflows, values
 //    down to 2^-26 of the maximum stay normal, and the product comes out at
 //    the tile's own scale.  The split-FP16 precision carries each operand as
