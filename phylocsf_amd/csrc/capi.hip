@@ -499,7 +499,8 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
       return set_err(c, PCSF_EINVAL, msg);
     for (int a = 0; a < 64; ++a)
       for (int k = 0; k < 64; ++k) QT[q * 4096 + k * 64 + a] = Q[a * 64 + k];
-    std::copy(m->eig[q].V.begin(), m->eig[q].V.end(), V.begin() + q * 4096);
+    for (int i = 0; i < 64; ++i)
+      for (int k = 0; k < 64; ++k) V[q * 4096 + k * 64 + i] = m->eig[q].V[i * 64 + k];
     std::copy(m->eig[q].Vinv.begin(), m->eig[q].Vinv.end(), Vi.begin() + q * 4096);
     std::copy(m->eig[q].lam.begin(), m->eig[q].lam.end(), lam.begin() + q * 64);
     for (int x = 0; x < 64; ++x) lnpi[q * 64 + x] = std::log(pi[q * 64 + x]);
@@ -519,7 +520,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
     return set_err(c, PCSF_EINVAL, "tree needs more than 6 live partial tiles");
 
   HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, dalloc(&m->d_V, V.size()));
+  HIP_TRY(c, dalloc(&m->d_VT, V.size()));
   HIP_TRY(c, dalloc(&m->d_Vinv, Vi.size()));
   HIP_TRY(c, dalloc(&m->d_lam, lam.size()));
   HIP_TRY(c, dalloc(&m->d_pi, m->pi.size()));
@@ -557,7 +558,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
   };
-  HIP_TRY(c, up(m->d_V, V.data(), V.size() * 8));
+  HIP_TRY(c, up(m->d_VT, V.data(), V.size() * 8));
   HIP_TRY(c, up(m->d_Vinv, Vi.data(), Vi.size() * 8));
   HIP_TRY(c, up(m->d_lam, lam.data(), lam.size() * 8));
   HIP_TRY(c, up(m->d_pi, m->pi.data(), m->pi.size() * 8));
@@ -578,7 +579,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
 
 void pcsf_model_destroy(pcsf_model* m) {
   if (!m) return;
-  dfree(m->d_V);
+  dfree(m->d_VT);
   dfree(m->d_Vinv);
   dfree(m->d_lam);
   dfree(m->d_pi);

@@ -85,7 +85,7 @@ struct pcsf_model {
   bool specialized = true;
   int first_gemm_br = kNoBranch;
   // device: model constants
-  double* d_V = nullptr;     // nm * 4096
+  double* d_VT = nullptr;    // nm * 4096, V transposed (VT[k][i] = V[i][k])
   double* d_Vinv = nullptr;  // nm * 4096
   double* d_lam = nullptr;   // nm * 64
   double* d_pi = nullptr;    // nm * 64
