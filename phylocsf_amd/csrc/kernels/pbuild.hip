@@ -28,202 +28,265 @@ namespace {
 
 constexpr double kTol = 1e-6;
 
-// LDS row stride in doubles: with 65, row i starts 2i banks along, so a wave
-// reading one column (64 rows) or a 4x16 block is free of bank conflicts
+// LDS row stride in doubles of the P^T copy the tangent GEMM reads
 constexpr int kS = 65;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// P = (V diag(e)) V^-1 with e = exp(lam t), on v_mfma_f64_16x16x4f64.  Wave w
-// makes rows 16w..16w+15 of P as four 16x16 tiles; at step s (k = 4s..4s+3)
-//   A[i][k] = V[16w + i][k] e[k]   in lane i + 16 (k % 4)
-//   B[k][j] = V^-1[k][16 ct + j]   in lane j + 16 (k % 4)
-//   D[i][j]                        in lane j + 16 (i % 4), element i / 4
-// VT is V transposed, so both operands load 16 consecutive doubles per k.
+// P^T = (V^-T diag(e)) V^T with e = exp(lam t), on v_mfma_f64_16x16x4f64.
+// Wave w makes rows 16w..16w+15 of P^T (columns of P) as four 16x16 tiles
+// ct; at step s (k = 4s..4s+3)
+//   A[i][k] = V^-1[k][16w + i] e[k]   in lane i + 16 (k % 4)
+//   B[k][j] = V[16 ct + j][k]          in lane j + 16 (k % 4)
+//   D[i][j]                            in lane j + 16 (i % 4), element i / 4
+// so element r of tile ct in lane (lr, lk) = (lane % 16, lane / 16) is
+//   P[a][b] with a = 16 ct + lr, b = 16 w + 4 r + lk.
+// In this layout every output below is written straight from the
+// accumulators: P^T rows are 16 consecutive entries per lane group and the
+// FP32 tile order is 64 consecutive floats per store.
+//
+// A workgroup makes the matrices of `nbw` consecutive branches of one set:
+// A and B do not depend on t, so each wave keeps its 16 A values (V^-1
+// unscaled) in registers and B = V^T sits in LDS for all of them; only e
+// changes from branch to branch.  B is stored with row k's 16-column halves
+// swapped when k is odd, so the two rows a half-wave reads per step fall in
+// disjoint banks.
 template <bool WITH_D>
-__global__ __launch_bounds__(256) void pbuild_kernel(
+__global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     const double* __restrict__ VT, const double* __restrict__ Vinv,
     const double* __restrict__ lam, const double* __restrict__ branch,
     const double* __restrict__ rho, const uint8_t* __restrict__ active, int nm, int nbr,
-    int nl, double* __restrict__ P64T, float* __restrict__ PT32, float* __restrict__ Pfrag, int fmt,
-    double* __restrict__ lnPcc, int* __restrict__ perr, double* __restrict__ perrval,
-    const double* __restrict__ QT, float* __restrict__ DT32, float* __restrict__ Dfrag) {
-  __shared__ double Ps[64 * kS];                // P, then fixed up
-  __shared__ double Ds[WITH_D ? 64 * kS : 1];   // b Q P
-  __shared__ double ex[64];
-  __shared__ double dinv[64];                   // 1 / P[c][c] after the fix-up
-  __shared__ double ptot[4][64], poff[4][64], pbad[4][64];
+    int nbw, int nl, double* __restrict__ P64T, float* __restrict__ PT32,
+    float* __restrict__ Pfrag, int fmt, double* __restrict__ lnPcc, int* __restrict__ perr,
+    double* __restrict__ perrval, const double* __restrict__ QT, float* __restrict__ DT32,
+    float* __restrict__ Dfrag) {
+  __shared__ double Bs[64 * 64];  // V^T, swizzled
+  __shared__ double ptot[4][64], poff[4][64], pbad[4][64];  // per wave, per row of P
   __shared__ int pjbad[4][64];
-  const int br = blockIdx.x, set = blockIdx.y, q = set % nm, tid = threadIdx.x;
+  __shared__ double pdiag[64], dinv[64];  // fixed-up diagonal and its inverse
+  __shared__ double Ps[WITH_D ? 64 * kS : 1];  // P^T, for the tangent GEMM
+  const int set = blockIdx.y, q = set % nm, tid = threadIdx.x;
   if (active && !active[set]) return;
-  const double t = rho[set] * branch[br];
-  if (tid < 64) ex[tid] = exp(t * lam[q * 64 + tid]);
-  if (t < 0.0) {
-    if (tid == 0 && atomicCAS(perr, 0, 4) == 0) {
-      perr[1] = br; perr[2] = set; perr[3] = 0; perr[4] = 0; *perrval = t;
-    }
-  }
+  const int br0 = blockIdx.x * nbw, br1 = min(nbr, br0 + nbw);
+  for (int e = tid; e < 4096; e += 256)
+    Bs[(e & ~63) | ((e & 63) ^ (((e >> 6) & 1) << 4))] = VT[(size_t)q * 4096 + e];
+  const double lm = lam[q * 64 + (tid & 63)];
   __syncthreads();
-  const int w = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
-  {
-    const double* A = VT + (size_t)q * 4096 + 16 * w + lr;
-    const double* B = Vinv + (size_t)q * 4096 + lr;
-    f64x4 acc[4] = {};
-#pragma unroll 4
-    for (int s = 0; s < 16; ++s) {
-      const int k = 4 * s + lk;
-      const double a = A[k * 64] * ex[k];
-#pragma unroll
-      for (int ct = 0; ct < 4; ++ct)
-        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, B[k * 64 + 16 * ct], acc[ct], 0, 0, 0);
-    }
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) Ps[(16 * w + 4 * r + lk) * kS + 16 * ct + lr] = acc[ct][r];
-  }
-  __syncthreads();
-  // Row fix-up.  Thread (row i = lane, quarter w) scans P[i][16w..16w+15]:
-  // entries below zero are an error beyond kTol and become 0; it keeps the
-  // row sum, the sum of the (clamped) off-diagonal entries and the first bad
-  // column; one thread per row then joins the quarters in column order.
-  {
-    const int i = lane;
-    double tot = 0.0, off = 0.0, bad = 0.0;
-    int jbad = -1;
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) {
-      const int j = 16 * w + jj;
-      double x = Ps[i * kS + j];
-      tot += x;
-      if (x < 0.0) {
-        if (fabs(x) > kTol && jbad < 0) { jbad = j; bad = x; }
-        x = 0.0;
-        Ps[i * kS + j] = 0.0;
-      }
-      if (j != i) off += x;
-    }
-    ptot[w][i] = tot;
-    poff[w][i] = off;
-    pbad[w][i] = bad;
-    pjbad[w][i] = jbad;
-  }
-  __syncthreads();
-  if (tid < 64) {
-    const int i = tid;
-    double tot = 0.0, off = 0.0, bad = 0.0;
-    int code = 0, jbad = 0;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      tot += ptot[v][i];
-      off += poff[v][i];
-      if (code == 0 && pjbad[v][i] >= 0) { code = 1; jbad = pjbad[v][i]; bad = pbad[v][i]; }
-    }
-    const double diag = 1.0 - off;
-    if (code == 0 && fabs(tot - 1.0) > kTol) { code = 2; bad = tot; }
-    if (code == 0 && !(diag <= 1.0 && diag > 0.0)) { code = 3; bad = diag; }
-    if (code != 0 && atomicCAS(perr, 0, code) == 0) {
-      perr[1] = br; perr[2] = set; perr[3] = i; perr[4] = jbad; *perrval = bad;
-    }
-    Ps[i * kS + i] = diag;
-    dinv[i] = 1.0 / diag;
-  }
-  __syncthreads();
-  const size_t sb = (size_t)set * nbr + br;
-  if (P64T) {
-    for (int e = tid; e < 4096; e += 256) P64T[sb * 4096 + e] = Ps[(e & 63) * kS + (e >> 6)];
-  }
-  // A leaf branch is read only through its leaf messages, an internal branch
-  // only through its tile: each writes just its own kind.
-  const bool leaf = br < nl;
-  if (leaf) {
-    for (int e = tid; e < 4096; e += 256) {
-      const int b = e >> 6, a = e & 63;  // transposed: [b][a] = P[a][b]
-      PT32[sb * (kLeafRows * 64) + e] = (float)(Ps[a * kS + b] * dinv[b]);
-    }
-    if (tid < 64) {
-      PT32[sb * (kLeafRows * 64) + 4096 + tid] = 1.0f;
-      lnPcc[sb * kLeafRows + tid] = log(Ps[tid * kS + tid]);
-    }
-    if (tid == 64) lnPcc[sb * kLeafRows + 64] = 0.0;
-  } else if (fmt == kFragF32) {
-    // fragment order of v_mfma_f32_16x16x4f32's A operand with the K index
-    // permuted to match the accumulator layout (prune_mfma.hpp):
-    //   e = ((mo*4 + mb)*64 + lane)*4 + v
-    //   row = 16*mo + lane%16, col = 16*mb + 4*(lane/16) + v
-    for (int e = tid; e < 4096; e += 256) {
-      const int v = e & 3, lane = (e >> 2) & 63, mb = (e >> 8) & 3, mo = e >> 10;
-      const int row = 16 * mo + (lane & 15);
-      const int col = 16 * mb + 4 * (lane >> 4) + v;
-      Pfrag[sb * kTileFloats + e] = (row == col) ? 0.0f : (float)Ps[row * kS + col];
-    }
-  } else {
-    // v_mfma_f32_16x16x32_{f16,bf16} A order, K permuted the same way:
-    //   e = ((mo*2 + st)*64 + lane)*8 + j
-    //   row = 16*mo + lane%16, col = 16*(2*st + j/4) + 4*(lane/16) + j%4
-    // From x = the FP32 value of the entry: hi = x rounded to nearest, lo =
-    // the residual x - hi (exact in FP32) rounded to nearest, so hi + lo is x
-    // to about 2^-22 relative.  FP16: x = offdiag(P) 2^13 and lo carries a
-    // further 2^13 (a normal number wherever hi is: the matrix cores read FP16
-    // subnormals as zero).
-    const float sc = (fmt == kFragF16) ? (float)(1 << kHalfExp) : 1.0f;
-    for (int e = tid; e < 4096; e += 256) {
-      const int j = e & 7, lane = (e >> 3) & 63, st = (e >> 9) & 1, mo = e >> 10;
-      const int row = 16 * mo + (lane & 15);
-      const int col = 16 * (2 * st + (j >> 2)) + 4 * (lane >> 4) + (j & 3);
-      // the off-diagonal of P, as in the FP32 tile: the products then sum
-      // nonnegative terms, and the diagonal P[a][a] - 1 (float 4096 on)
-      // joins in round-to-nearest VALU arithmetic
-      const float x = (row == col) ? 0.0f : (float)Ps[row * kS + col] * sc;
-      if (fmt == kFragF16) {
-        _Float16* h = reinterpret_cast<_Float16*>(Pfrag + sb * kTileFloats);
-        const _Float16 hi = (_Float16)x;
-        h[e] = hi;
-        h[4096 + e] = (_Float16)((x - (float)hi) * sc);
-      } else {
-        __bf16* h = reinterpret_cast<__bf16*>(Pfrag + sb * kTileFloats);
-        const __bf16 hi = (__bf16)x;
-        h[e] = hi;
-        h[4096 + e] = (__bf16)(x - (float)hi);
+  for (int br = br0; br < br1; ++br) {
+    // the lane indices are made opaque at every branch, so the compiler
+    // cannot hoist the many lane-dependent addresses and masks below out of
+    // the loop, where they would hold registers for its whole length
+    int tl = tid;
+    asm volatile("" : "+v"(tl));
+    const int w = tid >> 6, lane = tl & 63, lr = lane & 15, lk = lane >> 4;
+    const double* A = Vinv + (size_t)q * 4096 + 16 * w + lr;
+    const double t = rho[set] * branch[br];
+    if (t < 0.0) {
+      if (tid == 0 && atomicCAS(perr, 0, 4) == 0) {
+        perr[1] = br; perr[2] = set; perr[3] = 0; perr[4] = 0; *perrval = t;
       }
     }
-  }
-  if (!leaf && tid < 64) Pfrag[sb * kTileFloats + 4096 + tid] = (float)(Ps[tid * kS + tid] - 1.0);
-  if constexpr (WITH_D) {
-    // D = dP(rho b)/drho = b Q P(t) for the tangent kernel, on the same MFMA
-    // with A[i][k] = Q[16w + i][k] (QT[k][i]) and B[k][j] = P[k][16 ct + j]:
-    // leaf rows D[a][c] / P[c][c] (row 64 zero: `Marginalize does not depend
-    // on rho) for a leaf branch, the dense FP32 tile in the same A-operand
-    // order as Pfrag for an internal one
-    const double* A = QT + (size_t)q * 4096 + 16 * w + lr;
     f64x4 acc[4] = {};
+    {
+      // every wave makes e[lane] itself and takes e[k] from lane k
+      const double el = exp(t * lm);
+      const int sw = (lk & 1) << 4;
 #pragma unroll 4
-    for (int s = 0; s < 16; ++s) {
-      const int k = 4 * s + lk;
-      const double a = A[k * 64];
+      for (int s = 0; s < 16; ++s) {
+        const int k = 4 * s + lk;
+        const double a = A[k * 64] * __shfl(el, k);
 #pragma unroll
-      for (int ct = 0; ct < 4; ++ct)
-        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ps[k * kS + 16 * ct + lr], acc[ct], 0,
-                                                       0, 0);
+        for (int ct = 0; ct < 4; ++ct)
+          // column 16 ct + lr of B as stored in row k
+        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[k * 64 + ((16 * ct + lr) ^ sw)],
+                                                         acc[ct], 0, 0, 0);
+      }
     }
-    const double bl = branch[br];
+    // Row fix-up (rows a of P).  Each lane clamps its entries (below zero: an
+    // error beyond kTol, else 0) and sums them; the four lanes sharing lr join
+    // their sums and first bad column, the four waves then join in column order.
+    {
+      double tot[4], off[4], bv[4];
+      int jb[4];
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
+      for (int ct = 0; ct < 4; ++ct) {
+        tot[ct] = 0.0; off[ct] = 0.0; bv[ct] = 0.0; jb[ct] = 64;
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        Ds[(16 * w + 4 * r + lk) * kS + 16 * ct + lr] = bl * acc[ct][r];
+        for (int r = 0; r < 4; ++r) {
+          const int a = 16 * ct + lr, b = 16 * w + 4 * r + lk;
+          double x = acc[ct][r];
+          tot[ct] += x;
+          if (x < 0.0) {
+            if (fabs(x) > kTol && jb[ct] == 64) { jb[ct] = b; bv[ct] = x; }
+            x = 0.0;
+          }
+          acc[ct][r] = x;
+          if (a != b) off[ct] += x;
+        }
+#pragma unroll
+        for (int m = 16; m <= 32; m <<= 1) {
+          tot[ct] += __shfl_xor(tot[ct], m);
+          off[ct] += __shfl_xor(off[ct], m);
+          const int j2 = __shfl_xor(jb[ct], m);
+          const double b2 = __shfl_xor(bv[ct], m);
+          if (j2 < jb[ct]) { jb[ct] = j2; bv[ct] = b2; }
+        }
+      }
+      // lane (lr, lk) stores row a = 16 lk + lr = lane, i.e. tile ct = lk
+      const double t1 = lk == 0 ? tot[0] : lk == 1 ? tot[1] : lk == 2 ? tot[2] : tot[3];
+      const double o1 = lk == 0 ? off[0] : lk == 1 ? off[1] : lk == 2 ? off[2] : off[3];
+      const double b1 = lk == 0 ? bv[0] : lk == 1 ? bv[1] : lk == 2 ? bv[2] : bv[3];
+      const int j1 = lk == 0 ? jb[0] : lk == 1 ? jb[1] : lk == 2 ? jb[2] : jb[3];
+      ptot[w][lane] = t1;
+      poff[w][lane] = o1;
+      pbad[w][lane] = b1;
+      pjbad[w][lane] = j1;
+    }
     __syncthreads();
-    if (leaf) {
-      for (int e = tid; e < 4096; e += 256) {
-        const int c = e >> 6, a = e & 63;
-        DT32[sb * (kLeafRows * 64) + e] = (float)(Ds[a * kS + c] * dinv[c]);
+    if (tid < 64) {
+      const int i = tid;
+      double tot = 0.0, off = 0.0, bad = 0.0;
+      int code = 0, jbad = 0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        tot += ptot[v][i];
+        off += poff[v][i];
+        if (code == 0 && pjbad[v][i] < 64) { code = 1; jbad = pjbad[v][i]; bad = pbad[v][i]; }
       }
-      if (tid < 64) DT32[sb * (kLeafRows * 64) + 4096 + tid] = 0.0f;
+      const double diag = 1.0 - off;
+      if (code == 0 && fabs(tot - 1.0) > kTol) { code = 2; bad = tot; }
+      if (code == 0 && !(diag <= 1.0 && diag > 0.0)) { code = 3; bad = diag; }
+      if (code != 0 && atomicCAS(perr, 0, code) == 0) {
+        perr[1] = br; perr[2] = set; perr[3] = i; perr[4] = jbad; *perrval = bad;
+      }
+      pdiag[i] = diag;
+      dinv[i] = 1.0 / diag;
+    }
+    __syncthreads();
+    // the diagonal P[a][a] sits in tile ct = w, element r, of lane lr = 4 r + lk
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double d = pdiag[16 * w + 4 * r + lk];
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+        if (ct == w && lr == 4 * r + lk) acc[ct][r] = d;
+    }
+    const size_t sb = (size_t)set * nbr + br;
+    if (P64T) {
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          P64T[sb * 4096 + (16 * w + 4 * r + lk) * 64 + 16 * ct + lr] = acc[ct][r];
+    }
+    // A leaf branch is read only through its leaf messages, an internal branch
+    // only through its tile: each writes just its own kind.
+    const bool leaf = br < nl;
+    if (leaf) {
+      float* out = PT32 + sb * (kLeafRows * 64);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = 16 * w + 4 * r + lk;
+        const double di = dinv[b];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) out[b * 64 + 16 * ct + lr] = (float)(acc[ct][r] * di);
+      }
+      if (tid < 64) {
+        out[4096 + tid] = 1.0f;
+        lnPcc[sb * kLeafRows + tid] = log(pdiag[tid]);
+      }
+      if (tid == 64) lnPcc[sb * kLeafRows + 64] = 0.0;
     } else {
-      for (int e = tid; e < 4096; e += 256) {
-        const int v = e & 3, lane = (e >> 2) & 63, mb = (e >> 8) & 3, mo = e >> 10;
-        Dfrag[sb * 4096 + e] =
-            (float)Ds[(16 * mo + (lane & 15)) * kS + 16 * mb + 4 * (lane >> 4) + v];
+      // The tile: the off-diagonal of P (diagonal entries 0), so the products
+      // sum nonnegative terms, and the diagonal P[a][a] - 1 joins in
+      // round-to-nearest VALU arithmetic.  FP32: A-operand order of
+      // v_mfma_f32_16x16x4f32 with the K index permuted to match the
+      // accumulator layout (prune_mfma.hpp),
+      //   e = ((mo*4 + mb)*64 + L)*4 + v:  row 16 mo + L%16, col 16 mb + 4 (L/16) + v
+      // i.e. mo = ct, mb = w, L = lr + 16 r, v = lk.  Half formats: the
+      // v_mfma_f32_16x16x32_{f16,bf16} A order, K permuted the same way,
+      //   e = ((mo*2 + st)*64 + L)*8 + j:  row 16 mo + L%16,
+      //                                    col 16 (2 st + j/4) + 4 (L/16) + j%4
+      // i.e. st = w / 2, j = 4 (w % 2) + lk; hi and lo halves in two planes.
+      // From x = the FP32 value of the entry: hi = x rounded to nearest, lo =
+      // the residual x - hi (exact in FP32) rounded to nearest, so hi + lo is x
+      // to about 2^-22 relative.  FP16: x = offdiag(P) 2^13 and lo carries a
+      // further 2^13 (a normal number wherever hi is: the matrix cores read
+      // FP16 subnormals as zero).
+      float* out = Pfrag + sb * kTileFloats;
+      if (fmt == kFragF32) {
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const bool dg = (ct == w && lr == 4 * r + lk);
+            out[((ct * 4 + w) * 64 + lr + 16 * r) * 4 + lk] = dg ? 0.0f : (float)acc[ct][r];
+          }
+      } else {
+        const float sc = (fmt == kFragF16) ? (float)(1 << kHalfExp) : 1.0f;
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const bool dg = (ct == w && lr == 4 * r + lk);
+            const float x = dg ? 0.0f : (float)acc[ct][r] * sc;
+            const int e = ((ct * 2 + (w >> 1)) * 64 + lr + 16 * r) * 8 + 4 * (w & 1) + lk;
+            if (fmt == kFragF16) {
+              _Float16* h = reinterpret_cast<_Float16*>(out);
+              const _Float16 hi = (_Float16)x;
+              h[e] = hi;
+              h[4096 + e] = (_Float16)((x - (float)hi) * sc);
+            } else {
+              __bf16* h = reinterpret_cast<__bf16*>(out);
+              const __bf16 hi = (__bf16)x;
+              h[e] = hi;
+              h[4096 + e] = (__bf16)(x - (float)hi);
+            }
+          }
+      }
+      if (tid < 64) out[4096 + tid] = (float)(pdiag[tid] - 1.0);
+    }
+    if constexpr (WITH_D) {
+      // D = dP(rho b)/drho = b Q P(t) for the tangent kernel, as D^T = b P^T Q^T
+      // on the same MFMA with A[i][k] = P^T[16w + i][k] (from LDS) and
+      // B[k][j] = Q[16 ct + j][k] (QT[k][16 ct + j]); the result has the layout
+      // of P above.  Leaf rows D[a][c] / P[c][c] (row 64 zero: `Marginalize does
+      // not depend on rho) for a leaf branch, the dense FP32 tile in the order
+      // of the FP32 P tile for an internal one.
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Ps[(16 * w + 4 * r + lk) * kS + 16 * ct + lr] = acc[ct][r];
+      __syncthreads();
+      const double* B = QT + (size_t)q * 4096 + lr;
+      f64x4 dacc[4] = {};
+#pragma unroll 4
+      for (int s = 0; s < 16; ++s) {
+        const int k = 4 * s + lk;
+        const double a = Ps[(16 * w + lr) * kS + k];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+          dacc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, B[k * 64 + 16 * ct], dacc[ct], 0, 0, 0);
+      }
+      const double bl = branch[br];
+      if (leaf) {
+        float* out = DT32 + sb * (kLeafRows * 64);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int b = 16 * w + 4 * r + lk;
+          const double di = dinv[b];
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct)
+            out[b * 64 + 16 * ct + lr] = (float)((bl * dacc[ct][r]) * di);
+        }
+        if (tid < 64) out[4096 + tid] = 0.0f;
+      } else {
+        float* out = Dfrag + sb * 4096;
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            out[((ct * 4 + w) * 64 + lr + 16 * r) * 4 + lk] = (float)(bl * dacc[ct][r]);
       }
     }
   }
@@ -278,10 +341,13 @@ hipError_t launch_cherry_tables(const pcsf_model* m, const PSets& sets, int nset
 hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int fmt,
                          hipStream_t s) {
   if (nsets <= 0) return hipSuccess;
-  dim3 grid(m->nbr, nsets);
+  // several branches per workgroup once there are enough matrices to fill
+  // the device; one each for the few sets of a fixed-strategy model
+  const int nbw = ((size_t)nsets * m->nbr >= 16384) ? 8 : 1;
+  dim3 grid((m->nbr + nbw - 1) / nbw, nsets);
   auto* k = sets.d_Dfrag ? pbuild_kernel<true> : pbuild_kernel<false>;
   hipLaunchKernelGGL(k, grid, dim3(256), 0, s, m->d_VT, m->d_Vinv, m->d_lam,
-                     m->d_branch, sets.d_rho, sets.d_active, m->nm, m->nbr, m->nl,
+                     m->d_branch, sets.d_rho, sets.d_active, m->nm, m->nbr, nbw, m->nl,
                      sets.fp64 ? sets.d_P64T : nullptr, sets.d_PT32, sets.d_Pfrag, fmt,
                      sets.d_lnPcc, m->d_perr, m->d_perrval, m->d_QT, sets.d_DT32,
                      sets.d_Dfrag);
