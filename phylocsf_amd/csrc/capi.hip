@@ -1079,10 +1079,13 @@ int score_mle(pcsf_batch* b, int precision, std::vector<PairState>& st) {
 
 // Gradient maximiser of lpr over the tree scale for every (region, model)
 // pair: a safeguarded root search on h(x) = d lpr / d ln rho over
-// [ln 0.01, ln 10] (the reference's bounds): bracket from rho = 1 toward the
-// side the gradient points to, the boundary when the gradient keeps its
-// sign there, then Illinois regula falsi to |dx| <= 1e-7.  Each step is one
-// adjoint launch giving lpr and d lpr / d rho; a final forward gives elpr.
+// [ln 0.01, ln 10] (the reference's bounds): from rho = 1 a first step of
+// one unit of ln rho toward the side the gradient points to, then secant
+// Newton steps, the boundary when the gradient keeps its sign there, and
+// bisection when a step leaves the bracket or the bracket stops halving;
+// done when a step is below tol.  Each step is one gradient launch giving
+// lpr and d lpr / d rho; a final forward gives elpr.  st[k].evals counts the
+// launches of pair k, the final forward included.
 int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
