@@ -130,3 +130,26 @@ def test_published_examples_mle(ctx, examples_dir, prec):
             assert awin[0] < res["anc_comp"][bi] < awin[1], (fa, res["anc_comp"][bi])
         if where:
             assert tuple(rgns[bi]) == where, (fa, rgns[bi])
+
+
+def test_forward_per_region_many_sets(ctx, params_dir):
+    """Enough (region, model) sets that the P builder makes several branches per
+    workgroup (100vertebrates: 240 sets x 198 branches), with a branch count
+    that is not a multiple of that group; sampled regions against the oracle."""
+    _say("per-region rho forward, 100vertebrates, 120 regions")
+    m = P.PhyloCSFModel.from_paramset(ctx, "100vertebrates")
+    rng = np.random.default_rng(23)
+    n = 120
+    lengths = rng.integers(1, 9, n)
+    codes, off = m.simulate(lengths, rng.integers(0, 2, n), rng.uniform(0.3, 3, n),
+                            gap_p=0.1, n_p=0.01, seed=17)
+    rho = rng.uniform(0.05, 6.0, (n, 2))
+    om = _oracle_model(params_dir, "100vertebrates")
+    b = P.Batch(m, codes, off)
+    for prec, tol in ((P.PCSF_FP64, 1e-10), (P.PCSF_FP32, 1e-6)):
+        ll, el = b.forward(rho, prec, per_region=True)
+        for r in range(0, n, 7):
+            for q in range(2):
+                ref_l, ref_e = om.lpr(q, rho[r, q], codes[off[r]:off[r + 1]])
+                assert abs(ll[r, q] - ref_l) <= tol * max(1.0, abs(ref_l)), (prec, r, q)
+                assert abs(el[r, q] - ref_e) <= 10 * tol * max(1.0, abs(ref_e)), (prec, r, q)
