@@ -1,7 +1,8 @@
 // pbuild.hip -- FP64 transition matrices P(t) = V diag(exp(lam t)) V^-1 on the GPU.
 //
-// One workgroup per (branch, set); set s is model s % nm at tree scale
-// rho[s] and is skipped when active[s] == 0.  Reproduces Q.Diag.real_to_Pt
+// Set s is model s % nm at tree scale rho[s] and is skipped when
+// active[s] == 0; a workgroup makes the matrices of a few consecutive
+// branches of one set.  Reproduces Q.Diag.real_to_Pt
 // (lib/CamlPaml/Q.ml:211-249): entries below zero are an error when
 // |x| > 1e-6 and are set to 0 otherwise; the row sum must be within 1e-6 of
 // 1; the diagonal is replaced by 1 - sum(off-diagonal).  The first failure
@@ -28,10 +29,28 @@ namespace {
 
 constexpr double kTol = 1e-6;
 
-// LDS row stride in doubles of the P^T copy the tangent GEMM reads
-constexpr int kS = 65;
-
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Sum of v over the lanes xor 16 (row swap) and xor 32 (half swap): each swap
+// hands a lane its own value and its partner's, and the two are added in
+// either order to the same result, so all four lanes end with the same sum.
+__device__ __forceinline__ double lanes_sum(double v) {
+  {
+    const auto l = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false,
+                                                    false);
+    const auto h = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false,
+                                                    false);
+    v = __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+  }
+  {
+    const auto l = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false,
+                                                    false);
+    const auto h = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false,
+                                                    false);
+    v = __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+  }
+  return v;
+}
 
 // P^T = (V^-T diag(e)) V^T with e = exp(lam t), on v_mfma_f64_16x16x4f64.
 // Wave w makes rows 16w..16w+15 of P^T (columns of P) as four 16x16 tiles
@@ -43,14 +62,14 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 //   P[a][b] with a = 16 ct + lr, b = 16 w + 4 r + lk.
 // In this layout every output below is written straight from the
 // accumulators: P^T rows are 16 consecutive entries per lane group and the
-// FP32 tile order is 64 consecutive floats per store.
+// FP32 tile order is 64 consecutive floats per store.  With WITH_D the same
+// B also gives D^T = (V^-T diag(b lam e)) V^T, D = dP(rho b)/drho = b Q P.
 //
 // A workgroup makes the matrices of `nbw` consecutive branches of one set:
-// A and B do not depend on t, so each wave keeps its 16 A values (V^-1
-// unscaled) in registers and B = V^T sits in LDS for all of them; only e
-// changes from branch to branch.  B is stored with row k's 16-column halves
-// swapped when k is odd, so the two rows a half-wave reads per step fall in
-// disjoint banks.
+// A and B do not depend on t, so each wave reads its 16 A values (V^-1
+// unscaled) and B = V^T sits in LDS for all of them; only e changes from
+// branch to branch.  B is stored with row k's 16-column halves swapped when
+// k is odd, so the two rows a half-wave reads per step fall in disjoint banks.
 template <bool WITH_D>
 __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     const double* __restrict__ VT, const double* __restrict__ Vinv,
@@ -58,13 +77,11 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     const double* __restrict__ rho, const uint8_t* __restrict__ active, int nm, int nbr,
     int nbw, int nl, double* __restrict__ P64T, float* __restrict__ PT32,
     float* __restrict__ Pfrag, int fmt, double* __restrict__ lnPcc, int* __restrict__ perr,
-    double* __restrict__ perrval, const double* __restrict__ QT, float* __restrict__ DT32,
-    float* __restrict__ Dfrag) {
+    double* __restrict__ perrval, float* __restrict__ DT32, float* __restrict__ Dfrag) {
   __shared__ double Bs[64 * 64];  // V^T, swizzled
   __shared__ double ptot[4][64], poff[4][64], pbad[4][64];  // per wave, per row of P
   __shared__ int pjbad[4][64];
   __shared__ double pdiag[64], dinv[64];  // fixed-up diagonal and its inverse
-  __shared__ double Ps[WITH_D ? 64 * kS : 1];  // P^T, for the tangent GEMM
   const int set = blockIdx.y, q = set % nm, tid = threadIdx.x;
   if (active && !active[set]) return;
   const int br0 = blockIdx.x * nbw, br1 = min(nbr, br0 + nbw);
@@ -80,67 +97,98 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     asm volatile("" : "+v"(tl));
     const int w = tid >> 6, lane = tl & 63, lr = lane & 15, lk = lane >> 4;
     const double* A = Vinv + (size_t)q * 4096 + 16 * w + lr;
-    const double t = rho[set] * branch[br];
+    const double bl = branch[br];
+    const double t = rho[set] * bl;
     if (t < 0.0) {
       if (tid == 0 && atomicCAS(perr, 0, 4) == 0) {
         perr[1] = br; perr[2] = set; perr[3] = 0; perr[4] = 0; *perrval = t;
       }
     }
     f64x4 acc[4] = {};
+    f64x4 dacc[WITH_D ? 4 : 1] = {};
     {
       // every wave makes e[lane] itself and takes e[k] from lane k
       const double el = exp(t * lm);
+      const double dl = WITH_D ? bl * lm * el : 0.0;
       const int sw = (lk & 1) << 4;
 #pragma unroll 4
       for (int s = 0; s < 16; ++s) {
         const int k = 4 * s + lk;
-        const double a = A[k * 64] * __shfl(el, k);
+        const double v = A[k * 64];
+        const double a = v * __shfl(el, k);
+        double ad = 0.0;
+        if constexpr (WITH_D) ad = v * __shfl(dl, k);
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct)
+        for (int ct = 0; ct < 4; ++ct) {
           // column 16 ct + lr of B as stored in row k
-        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[k * 64 + ((16 * ct + lr) ^ sw)],
-                                                         acc[ct], 0, 0, 0);
+          const double bv = Bs[k * 64 + ((16 * ct + lr) ^ sw)];
+          acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[ct], 0, 0, 0);
+          if constexpr (WITH_D)
+            dacc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, bv, dacc[ct], 0, 0, 0);
+        }
       }
     }
-    // Row fix-up (rows a of P).  Each lane clamps its entries (below zero: an
-    // error beyond kTol, else 0) and sums them; the four lanes sharing lr join
-    // their sums and first bad column, the four waves then join in column order.
+    // Row fix-up (rows a of P).  An entry below -kTol fails the build: only
+    // then is each row's first such column searched for (the four lanes
+    // sharing lr, then the four waves in column order).  Otherwise every
+    // lane clamps its entries at 0 and sums them, raw and clamped, with the
+    // clamped diagonal taken out of the latter; the lanes sharing lr and then
+    // the waves join the sums.
     {
-      double tot[4], off[4], bv[4];
-      int jb[4];
+      double mn = acc[0][0];
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) mn = fmin(mn, acc[ct][r]);
+      if (__any(mn < -kTol)) {
+        double bv[4];
+        int jb[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+          bv[ct] = 0.0;
+          jb[ct] = 64;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double x = acc[ct][r];
+            if (x < -kTol && jb[ct] == 64) { jb[ct] = 16 * w + 4 * r + lk; bv[ct] = x; }
+          }
+#pragma unroll
+          for (int m = 16; m <= 32; m <<= 1) {
+            const int j2 = __shfl_xor(jb[ct], m);
+            const double b2 = __shfl_xor(bv[ct], m);
+            if (j2 < jb[ct]) { jb[ct] = j2; bv[ct] = b2; }
+          }
+        }
+        // lane (lr, lk) stores row a = 16 lk + lr = lane, i.e. tile ct = lk
+        pbad[w][lane] = lk == 0 ? bv[0] : lk == 1 ? bv[1] : lk == 2 ? bv[2] : bv[3];
+        pjbad[w][lane] = lk == 0 ? jb[0] : lk == 1 ? jb[1] : lk == 2 ? jb[2] : jb[3];
+      } else {
+        pjbad[w][lane] = 64;
+      }
+      double tot[4], off[4];
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
-        tot[ct] = 0.0; off[ct] = 0.0; bv[ct] = 0.0; jb[ct] = 64;
+        tot[ct] = 0.0;
+        off[ct] = 0.0;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int a = 16 * ct + lr, b = 16 * w + 4 * r + lk;
-          double x = acc[ct][r];
+          const double x = acc[ct][r];
           tot[ct] += x;
-          if (x < 0.0) {
-            if (fabs(x) > kTol && jb[ct] == 64) { jb[ct] = b; bv[ct] = x; }
-            x = 0.0;
-          }
-          acc[ct][r] = x;
-          if (a != b) off[ct] += x;
+          const double xc = fmax(x, 0.0);
+          acc[ct][r] = xc;
+          off[ct] += xc;
         }
+        // the diagonal P[a][a] sits in tile ct = w, element r, of lane lr = 4 r + lk
+        if (ct == w) {
 #pragma unroll
-        for (int m = 16; m <= 32; m <<= 1) {
-          tot[ct] += __shfl_xor(tot[ct], m);
-          off[ct] += __shfl_xor(off[ct], m);
-          const int j2 = __shfl_xor(jb[ct], m);
-          const double b2 = __shfl_xor(bv[ct], m);
-          if (j2 < jb[ct]) { jb[ct] = j2; bv[ct] = b2; }
+          for (int r = 0; r < 4; ++r)
+            if (lr == 4 * r + lk) off[ct] -= acc[ct][r];
         }
+        tot[ct] = lanes_sum(tot[ct]);
+        off[ct] = lanes_sum(off[ct]);
       }
-      // lane (lr, lk) stores row a = 16 lk + lr = lane, i.e. tile ct = lk
-      const double t1 = lk == 0 ? tot[0] : lk == 1 ? tot[1] : lk == 2 ? tot[2] : tot[3];
-      const double o1 = lk == 0 ? off[0] : lk == 1 ? off[1] : lk == 2 ? off[2] : off[3];
-      const double b1 = lk == 0 ? bv[0] : lk == 1 ? bv[1] : lk == 2 ? bv[2] : bv[3];
-      const int j1 = lk == 0 ? jb[0] : lk == 1 ? jb[1] : lk == 2 ? jb[2] : jb[3];
-      ptot[w][lane] = t1;
-      poff[w][lane] = o1;
-      pbad[w][lane] = b1;
-      pjbad[w][lane] = j1;
+      ptot[w][lane] = lk == 0 ? tot[0] : lk == 1 ? tot[1] : lk == 2 ? tot[2] : tot[3];
+      poff[w][lane] = lk == 0 ? off[0] : lk == 1 ? off[1] : lk == 2 ? off[2] : off[3];
     }
     __syncthreads();
     if (tid < 64) {
@@ -247,28 +295,10 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
       if (tid < 64) out[4096 + tid] = (float)(pdiag[tid] - 1.0);
     }
     if constexpr (WITH_D) {
-      // D = dP(rho b)/drho = b Q P(t) for the tangent kernel, as D^T = b P^T Q^T
-      // on the same MFMA with A[i][k] = P^T[16w + i][k] (from LDS) and
-      // B[k][j] = Q[16 ct + j][k] (QT[k][16 ct + j]); the result has the layout
-      // of P above.  Leaf rows D[a][c] / P[c][c] (row 64 zero: `Marginalize does
-      // not depend on rho) for a leaf branch, the dense FP32 tile in the order
-      // of the FP32 P tile for an internal one.
-#pragma unroll
-      for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Ps[(16 * w + 4 * r + lk) * kS + 16 * ct + lr] = acc[ct][r];
-      __syncthreads();
-      const double* B = QT + (size_t)q * 4096 + lr;
-      f64x4 dacc[4] = {};
-#pragma unroll 4
-      for (int s = 0; s < 16; ++s) {
-        const int k = 4 * s + lk;
-        const double a = Ps[(16 * w + lr) * kS + k];
-#pragma unroll
-        for (int ct = 0; ct < 4; ++ct)
-          dacc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, B[k * 64 + 16 * ct], dacc[ct], 0, 0, 0);
-      }
-      const double bl = branch[br];
+      // D = dP(rho b)/drho = b Q P(t) for the tangent kernel, in the layout
+      // of P: leaf rows D[a][c] / P[c][c] (row 64 zero: `Marginalize does not
+      // depend on rho) for a leaf branch, the dense FP32 tile in the order of
+      // the FP32 P tile for an internal one
       if (leaf) {
         float* out = DT32 + sb * (kLeafRows * 64);
 #pragma unroll
@@ -276,8 +306,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
           const int b = 16 * w + 4 * r + lk;
           const double di = dinv[b];
 #pragma unroll
-          for (int ct = 0; ct < 4; ++ct)
-            out[b * 64 + 16 * ct + lr] = (float)((bl * dacc[ct][r]) * di);
+          for (int ct = 0; ct < 4; ++ct) out[b * 64 + 16 * ct + lr] = (float)(dacc[ct][r] * di);
         }
         if (tid < 64) out[4096 + tid] = 0.0f;
       } else {
@@ -286,7 +315,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
         for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            out[((ct * 4 + w) * 64 + lr + 16 * r) * 4 + lk] = (float)(bl * dacc[ct][r]);
+            out[((ct * 4 + w) * 64 + lr + 16 * r) * 4 + lk] = (float)dacc[ct][r];
       }
     }
   }
@@ -349,7 +378,7 @@ hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int 
   hipLaunchKernelGGL(k, grid, dim3(256), 0, s, m->d_VT, m->d_Vinv, m->d_lam,
                      m->d_branch, sets.d_rho, sets.d_active, m->nm, m->nbr, nbw, m->nl,
                      sets.fp64 ? sets.d_P64T : nullptr, sets.d_PT32, sets.d_Pfrag, fmt,
-                     sets.d_lnPcc, m->d_perr, m->d_perrval, m->d_QT, sets.d_DT32,
+                     sets.d_lnPcc, m->d_perr, m->d_perrval, sets.d_DT32,
                      sets.d_Dfrag);
   return hipGetLastError();
 }
