@@ -111,6 +111,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
       const double el = exp(t * lm);
       const double dl = WITH_D ? bl * lm * el : 0.0;
       const int sw = (lk & 1) << 4;
+      __builtin_amdgcn_s_setprio(1);
 #pragma unroll 4
       for (int s = 0; s < 16; ++s) {
         const int k = 4 * s + lk;
@@ -127,6 +128,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
             dacc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, bv, dacc[ct], 0, 0, 0);
         }
       }
+      __builtin_amdgcn_s_setprio(0);
     }
     // Row fix-up (rows a of P).  An entry below -kTol fails the build: only
     // then is each row's first such column searched for (the four lanes

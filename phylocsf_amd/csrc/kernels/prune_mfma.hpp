@@ -484,6 +484,7 @@ struct Walk {
       for (int c = 0; c < G; ++c)
 #pragma unroll
         for (int mo = 0; mo < 4; ++mo) acc[c][mo] = (f32x4)(0.0f);
+      if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
         f32x4 A[4];
@@ -500,6 +501,7 @@ struct Walk {
               acc[c][mo] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mo][v], bv, acc[c][mo], 0, 0, 0);
           }
       }
+      if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
       // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, the diagonal term in
       // round-to-nearest VALU arithmetic
 #pragma unroll
