@@ -22,7 +22,11 @@
 //         zero) for v_mfma_f32_16x16x4f32, or hi and lo halves of the same
 //         off-diagonal (FP16 scaled by 2^kHalfExp, or BF16) for
 //         v_mfma_f32_16x16x32_{f16,bf16};
-//         followed by the 64 FP32 values P[a][a] - 1 (read by the FP32 path).
+//         followed by the 64 FP32 values P[a][a] - 1 (the diagonal term, which
+//         every precision adds in VALU arithmetic).
+//   DT32, Dfrag  (tangent sets only) D = dP(rho b)/drho = b Q P in the layouts
+//         of PT32 (divided by P[c][c]; row 64 zero) and of the FP32 tile
+//         (dense), for leaf and internal branches respectively.
 #include "../internal.hpp"
 
 namespace {
