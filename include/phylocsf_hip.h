@@ -193,8 +193,9 @@ int pcsf_score(pcsf_batch *b, int strategy, int precision, double *score, double
  * this batch at tree scale 1,
  * `reps` launches each, keeps the fastest on the batch's model for later
  * forwards at that precision and returns the per-shape milliseconds in
- * ms_out[PCSF_N_VARIANTS].  No counterpart in the reference (a tuning step
- * of this implementation). */
+ * ms_out[PCSF_N_VARIANTS].  Until then a model uses shape 2 if its tree has
+ * generated programs (see pcsf_model_specialized), else shape 1.  No
+ * counterpart in the reference (a tuning step of this implementation). */
 #define PCSF_N_VARIANTS 7
 int pcsf_autotune(pcsf_batch *b, int precision, int reps, double *ms_out, int *chosen);
 /* Select a launch shape by index (0..PCSF_N_VARIANTS-1) for one MFMA
@@ -202,7 +203,7 @@ int pcsf_autotune(pcsf_batch *b, int precision, int reps, double *ms_out, int *c
 int pcsf_set_variant(pcsf_model *m, int precision, int variant);
 /* Tree-specialised kernels: the library carries straight-line op programs
  * generated at build time for the shipped trees; a model whose tree matches
- * one uses it (FP32 and split-FP16 GEMMs, launch shapes 1, 2 and 4) instead
+ * one uses it (FP32 and split-FP16 GEMMs, launch shapes 1, 2, 4 and 6) instead
  * of the op-program interpreter.  enable = 1 / 0 allows / forbids them
  * (-1 leaves the setting); *available (may be NULL) gets 1 if the plain
  * program is specialised, | 2 if the cherry-table program is.  No

@@ -516,6 +516,11 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   m->n_gemm = pg.n_gemm;
   m->tree_p = find_tree_program(m->prog, false);
   m->tree_t = find_tree_program(m->prog_t, true);
+  // default launch shape until pcsf_autotune: three waves per SIMD for the
+  // straight-line programs (the fastest shape on every shipped tree measured),
+  // two for the interpreter, whose switch needs more registers
+  if (m->tree_p && m->tree_t)
+    for (int& v : m->mfma_variant) v = 2;
   if (m->slots > kMaxSlots)
     return set_err(c, PCSF_EINVAL, "tree needs more than 6 live partial tiles");
 

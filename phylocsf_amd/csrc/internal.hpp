@@ -77,7 +77,8 @@ struct pcsf_model {
   int32_t* d_cherries = nullptr;  // node, left leaf, right leaf per table
   int slots = 0;
   int n_gemm = 0;
-  int mfma_variant[5] = {1, 1, 1, 1, 1};  // launch shape per precision (pcsf_autotune)
+  int mfma_variant[5] = {1, 1, 1, 1, 1};  // launch shape per precision (pcsf_autotune;
+                                          // 2 for trees with generated programs)
   // tree-specialised programs matching prog / prog_t (nullptr: none), and
   // whether launches may use them
   const void* tree_p = nullptr;
