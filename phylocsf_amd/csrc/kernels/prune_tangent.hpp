@@ -168,6 +168,9 @@ struct TWalk {
     }
     const float* Ab = Pbuf[kg & 1];
     const float* Db = Dbuf[kg & 1];
+    // P a' and D a stay in separate accumulators: summed in one (the MFMA's
+    // FP32 accumulation rounds toward zero) the gradients measured 2-4x less
+    // accurate against the FP64 adjoint
     f32x4 as[4], at[4], ad[4];
 #pragma unroll
     for (int mo = 0; mo < 4; ++mo) as[mo] = at[mo] = ad[mo] = (f32x4)(0.0f);
