@@ -98,7 +98,11 @@ int pcsf_program(int n_nodes, const int32_t *children, int with_tables, int32_t 
  * Q_m = (S_m o pi_m)/scale_m, root prior pi_m, branch lengths rho * b.
  * Eigendecomposes each Q_m (FP64, host) and uploads it.
  * S is n_models*64*64, pi is n_models*64, branch has n_nodes entries (the
- * root's is ignored). */
+ * root's is ignored).  Every S must be symmetric (a reversible model, as
+ * ECM.import_parameters builds it); PCSF_EINVAL otherwise.
+ * ctx may be NULL: a host-only model, for pcsf_simulate,
+ * pcsf_simulate_nucleotides and pcsf_model_info (no device memory; the
+ * device entry points refuse it with PCSF_EINVAL). */
 int pcsf_model_create(pcsf_ctx *ctx, int n_nodes, const int32_t *children,
                       const double *branch, int n_models, const double *S, const double *pi,
                       pcsf_model **out);
@@ -117,7 +121,8 @@ int pcsf_model_transition(pcsf_model *m, int model, double rho, double *P_out);
 /* codes: n_cols x n_leaves (row per codon column, as PhyloCSF.pleaves
  * yields them, src/PhyloCSF.ml:219-246), host memory; uploaded once and
  * kept leaf-major in HBM.  region_offsets[n_regions+1]: columns of region r
- * are [off[r], off[r+1]). */
+ * are [off[r], off[r+1]).  PCSF_EINVAL if a code is neither 0..63 nor
+ * PCSF_MARGINALIZE (checked on the device while the codes are transposed). */
 int pcsf_batch_create(pcsf_model *m, const uint8_t *codes, int64_t n_cols,
                       const int64_t *region_offsets, int32_t n_regions, pcsf_batch **out);
 /* A batch of every reading frame of packed nucleotide regions: PhyloCSF's
@@ -136,6 +141,12 @@ int pcsf_batch_create_frames(pcsf_model *m, const uint8_t *nuc, int64_t n_nuc,
                              pcsf_batch **out);
 void pcsf_batch_destroy(pcsf_batch *b);
 int64_t pcsf_batch_columns(const pcsf_batch *b);
+/* The batch's leaf codes as the kernels read them, back on the host:
+ * codes (may be NULL) gets n_cols x n_leaves bytes, a row per codon column as
+ * PhyloCSF.pleaves yields them (src/PhyloCSF.ml:219-246); region_offsets
+ * (may be NULL) gets the n_regions + 1 column offsets (for a frame batch,
+ * region r * frames + f).  Lets a caller check GPU-built codes byte for byte. */
+int pcsf_batch_codes(pcsf_batch *b, uint8_t *codes, int64_t *region_offsets);
 
 /* Forward: for every region r and model m,
  *   loglik[r*n_models+m] = sum_col ln z_col          (lpr_leaves)
@@ -211,7 +222,10 @@ int pcsf_set_variant(pcsf_model *m, int precision, int variant);
 int pcsf_model_specialized(pcsf_model *m, int enable, int *available);
 
 /* Device pointers of the last forward's per-region results (FP64,
- * n_regions*n_models each) and the kernel times when timing is enabled. */
+ * n_regions*n_models each) and the kernel times when timing is enabled.
+ * pcsf_backward does not change them (its passes reduce into buffers of
+ * their own); pcsf_score with an mle strategy runs forwards of its own, so
+ * afterwards they hold its last evaluation's values. */
 int pcsf_batch_device_results(pcsf_batch *b, const double **d_loglik, const double **d_elpr);
 int pcsf_batch_last_timing(const pcsf_batch *b, double *prune_ms, double *reduce_ms,
                            double *pbuild_ms);

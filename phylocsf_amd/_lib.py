@@ -32,7 +32,7 @@ EXPORTED = [
     "pcsf_backward", "pcsf_batch_device_results", "pcsf_batch_last_timing",
     "pcsf_lik_columns", "pcsf_simulate", "pcsf_autotune", "pcsf_set_variant", "pcsf_score",
     "pcsf_model_specialized", "pcsf_batch_create_frames", "pcsf_simulate_nucleotides",
-    "pcsf_program",
+    "pcsf_program", "pcsf_batch_codes",
 ]
 PCSF_STRATEGY_FIXED = 0
 PCSF_STRATEGY_MLE = 1
@@ -105,6 +105,7 @@ def lib():
     L.pcsf_batch_destroy.restype = None
     L.pcsf_batch_columns.argtypes = [vp]
     L.pcsf_batch_columns.restype = C.c_int64
+    L.pcsf_batch_codes.argtypes = [vp, u8p, i64p]
     L.pcsf_forward.argtypes = [vp, dp, C.c_int, C.c_int, dp, dp, C.c_int]
     L.pcsf_backward.argtypes = [vp, dp, dp, dp]
     L.pcsf_batch_device_results.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]

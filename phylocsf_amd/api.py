@@ -156,11 +156,21 @@ class Context:
             pass
 
 
+class _HostOnly:
+    """Stands in for a Context when a model is host-only (no device)."""
+    h = None
+
+    def check(self, rc):
+        if rc:
+            raise_for(rc, "host-only model: the call needs a device context")
+
+
 class PhyloCSFModel:
     """PhyloCSFModel.make over one tree and n ECMs (0 = coding, 1 = noncoding)."""
 
-    def __init__(self, ctx: Context, tree: Tree, ecms: Sequence):
-        self.ctx = ctx
+    def __init__(self, ctx: Optional[Context], tree: Tree, ecms: Sequence):
+        """ctx None: a host-only model (simulation and model facts only)."""
+        self.ctx = ctx if ctx is not None else _HostOnly()
         self.tree = tree
         S = np.ascontiguousarray(np.stack([e[0] for e in ecms]), dtype=np.float64)
         pi = np.ascontiguousarray(np.stack([e[1] for e in ecms]), dtype=np.float64)
@@ -170,12 +180,12 @@ class PhyloCSFModel:
         br = np.ascontiguousarray(tree.branch, dtype=np.float64).copy()
         br[-1] = 0.0  # the root branch is unused
         h = C.c_void_p()
-        ctx.check(lib().pcsf_model_create(ctx.h, tree.n, _i32(ch), _dp(br), self.nm, _dp(S),
-                                          _dp(pi), C.byref(h)))
+        self.ctx.check(lib().pcsf_model_create(self.ctx.h, tree.n, _i32(ch), _dp(br), self.nm,
+                                               _dp(S), _dp(pi), C.byref(h)))
         self.h = h
 
     @classmethod
-    def from_paramset(cls, ctx: Context, paramset: str, species=None):
+    def from_paramset(cls, ctx: Optional[Context], paramset: str, species=None):
         prefix = paramset_prefix(paramset)
         tree = newick_tree(open(prefix + ".nh").read(), species)
         return cls(ctx, tree, [import_parameters(prefix + "_coding.ECM"),
@@ -262,6 +272,11 @@ class Batch:
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         if codes.ndim != 2 or codes.shape[1] != model.tree.nl:
             raise InvalidArgument(-1, "codes must be [n_cols, n_leaves]")
+        # a code is a codon index 0..63 or PCSF_MARGINALIZE (255); anything
+        # else would index past the kernels' 65-row leaf tables
+        if codes.size and (codes.max() > 63) and ((codes > 63) & (codes != PCSF_MARGINALIZE)).any():
+            raise InvalidArgument(-1, "pcsf_batch_create: a leaf code is neither a codon index "
+                                      "(0..63) nor PCSF_MARGINALIZE (255)")
         self.n_regions = len(offsets) - 1
         self.n_cols = codes.shape[0]
         h = C.c_void_p()
@@ -297,6 +312,14 @@ class Batch:
         codes = (np.concatenate(regions, axis=0) if regions
                  else np.zeros((0, model.tree.nl), np.uint8))
         return cls(model, codes, off)
+
+    def codes(self):
+        """The leaf codes the kernels read, back from the device:
+        -> (codes [n_cols, n_leaves] uint8, region offsets [R+1])."""
+        codes = np.zeros((self.n_cols, self.model.tree.nl), np.uint8)
+        off = np.zeros(self.n_regions + 1, np.int64)
+        self.model.ctx.check(lib().pcsf_batch_codes(self.h, _u8(codes), _i64(off)))
+        return codes, off
 
     def forward(self, rho=1.0, precision=PCSF_FP32, per_region=False):
         """-> (loglik [R, nm], elpr [R, nm]): lpr_leaves for every region/model."""

@@ -28,10 +28,20 @@ int set_err(pcsf_ctx* c, int code, const std::string& msg) {
                      std::string(#expr) + ": " + hipGetErrorString(e_));                \
   } while (0)
 
+// PCSF_DEVICE_ALLOC_LIMIT=<bytes>: any single device allocation larger than
+// that fails as if the device were out of memory (read at every allocation;
+// lets a test drive the library's recovery from a failed allocation).
+bool alloc_over_limit(size_t bytes) {
+  const char* lim = std::getenv("PCSF_DEVICE_ALLOC_LIMIT");
+  if (!lim || !*lim) return false;
+  return bytes > std::strtoull(lim, nullptr, 10);
+}
+
 template <typename T>
 hipError_t dalloc(T** p, size_t count) {
   *p = nullptr;
   if (count == 0) return hipSuccess;
+  if (alloc_over_limit(count * sizeof(T))) return hipErrorOutOfMemory;
   return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
 }
 
@@ -42,6 +52,8 @@ void dfree(T*& p) {
 }
 
 bool rho_valid(double r) { return r > 0.0 && std::isfinite(r); }
+
+void free_psets(PSets& ps);
 
 hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0,
                        bool tangent = false) {
@@ -64,8 +76,13 @@ hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0,
     chk(dalloc(&ps.d_ctab, (size_t)cap * nctab * kTabCodes * kTabCodes * 64));
     chk(dalloc(&ps.d_clnt, (size_t)cap * nctab * kTabCodes * kTabCodes));
   }
+  if (e != hipSuccess) {
+    free_psets(ps);  // nothing half-allocated survives: cap stays 0
+    return e;
+  }
   ps.cap = cap;
   ps.fp64 = fp64;
+  ps.fmt = -1;
   return e;
 }
 
@@ -81,14 +98,54 @@ void free_psets(PSets& ps) {
   dfree(ps.d_DT32);
   dfree(ps.d_Dfrag);
   ps.cap = 0;
+  ps.fp64 = false;
+  ps.fmt = -1;
+}
+
+// The reference's message for a P(t) failure recorded by the P builder as
+// key (kernels/pbuild.hip: ((set * nbr + br) * 64 + row) * 65 + col, then
+// two bits of the check).  The failing matrix is rebuilt on the host in FP64
+// (Q.Diag.real_to_Pt, lib/CamlPaml/Q.ml:211-249), which also gives the
+// offending value; should the host copy pass where the device's failed (a
+// rounding-level difference), the device's position is reported.
+int pbuild_failure(pcsf_model* m, unsigned long long key, const double* rho_host) {
+  pcsf_ctx* c = m->ctx;
+  const int check = (int)(key & 3);
+  unsigned long long pos = key >> 2;
+  const int col = (int)(pos % 65);
+  pos /= 65;
+  const int row = (int)(pos % 64);
+  pos /= 64;
+  const int br = (int)(pos % (unsigned long long)m->nbr);
+  const int set = (int)(pos / (unsigned long long)m->nbr);
+  const double t = rho_host[set] * m->branch[br];
+  if (check == 0) return set_err(c, PCSF_EINVAL, "CamlPaml.Q.to_Pt");
+  std::vector<double> P(4096);
+  std::string msg;
+  if (!pcsf::transition_host(m->eig[set % m->nm], t, 1e-6, P.data(), &msg)) {
+    if (msg.rfind("Assert_failure", 0) == 0) msg = "Assert_failure (Q.ml:245)";
+    return set_err(c, PCSF_EFAIL, msg);
+  }
+  char buf[256];
+  if (check == 1)
+    std::snprintf(buf, sizeof buf, "CamlPaml.Q.substition_matrix: expm(%.2e*Q)[%d,%d] < 0", t,
+                  row, col);
+  else if (check == 2)
+    std::snprintf(buf, sizeof buf, "CamlPaml.Q.substitution matrix: sum(expm(%.2e*Q)[%d,] > 1",
+                  t, row);
+  else
+    std::snprintf(buf, sizeof buf, "Assert_failure (Q.ml:245)");
+  return set_err(c, PCSF_EFAIL, buf);
 }
 
 // Runs the P builder over `nsets` sets whose rho/active are already on the
-// device; maps a recorded failure to the reference's message.
+// device.  A failure comes back as the key of the first failing (set,
+// branch, row) in the reference's order; its message is made on the host
+// from the same P(t) built in FP64 by Q.Diag.real_to_Pt's rules.
 int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int fmt,
                float* ms_out) {
   pcsf_ctx* c = m->ctx;
-  HIP_TRY(c, hipMemsetAsync(m->d_perr, 0, sizeof(int) * 8, c->stream));
+  HIP_TRY(c, hipMemsetAsync(m->d_perr, 0xff, sizeof(unsigned long long), c->stream));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing && ms_out) {
     HIP_TRY(c, hipEventCreate(&e0));
@@ -98,11 +155,8 @@ int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int 
   ps.fmt = -1;
   HIP_TRY(c, launch_pbuild(m, ps, nsets, fmt, c->stream));
   if (e1) HIP_TRY(c, hipEventRecord(e1, c->stream));
-  int perr[8];
-  double perrval = 0;
-  HIP_TRY(c, hipMemcpyAsync(perr, m->d_perr, sizeof perr, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipMemcpyAsync(&perrval, m->d_perrval, sizeof(double), hipMemcpyDeviceToHost,
-                            c->stream));
+  unsigned long long key = ~0ull;
+  HIP_TRY(c, hipMemcpyAsync(&key, m->d_perr, sizeof key, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (e0) {
     float ms = 0;
@@ -111,28 +165,7 @@ int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
-  if (perr[0] != 0) {
-    char buf[256];
-    const int br = perr[1], set = perr[2];
-    const double t = rho_host[set] * m->branch[br];
-    switch (perr[0]) {
-      case 1:
-        std::snprintf(buf, sizeof buf,
-                      "CamlPaml.Q.substition_matrix: expm(%.2e*Q)[%d,%d] = %e < 0", t, perr[3],
-                      perr[4], perrval);
-        break;
-      case 2:
-        std::snprintf(buf, sizeof buf,
-                      "CamlPaml.Q.substitution matrix: sum(expm(%.2e*Q)[%d,] = %e > 1", t,
-                      perr[3], perrval);
-        break;
-      case 4:
-        return set_err(c, PCSF_EINVAL, "CamlPaml.PhyloModel.make");
-      default:
-        std::snprintf(buf, sizeof buf, "Assert_failure (Q.ml:245) row %d", perr[3]);
-    }
-    return set_err(c, PCSF_EFAIL, buf);
-  }
+  if (key != ~0ull) return pbuild_failure(m, key, rho_host);
   ps.fmt = fmt;
   return PCSF_OK;
 }
@@ -177,16 +210,33 @@ int ensure_aligned(pcsf_batch* b) {
     const int64_t len = b->offsets[r + 1] - b->offsets[r];
     for (int64_t k = 0; k < (len + kAlignCols - 1) / kAlignCols; ++k) br[al.block0[r] + k] = r;
   }
-  HIP_TRY(c, dalloc(&al.d_codes, (size_t)m->nl * al.ld));
-  HIP_TRY(c, dalloc(&al.d_block_region, br.size()));
-  HIP_TRY(c, dalloc(&al.d_astart, al.astart.size()));
-  HIP_TRY(c, dalloc(&b->d_alogz, (size_t)m->nm * al.ld));
-  HIP_TRY(c, dalloc(&b->d_acolelpr, (size_t)m->nm * al.ld));
-  HIP_TRY(c, hipMemcpy(al.d_block_region, br.data(), br.size() * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(al.d_astart, al.astart.data(), al.astart.size() * 8,
-                       hipMemcpyHostToDevice));
-  HIP_TRY(c, launch_align_codes(b, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) {
+    if (e == hipSuccess) e = x;
+  };
+  chk(dalloc(&al.d_codes, (size_t)m->nl * al.ld));
+  chk(dalloc(&al.d_block_region, br.size()));
+  chk(dalloc(&al.d_astart, al.astart.size()));
+  chk(dalloc(&b->d_alogz, (size_t)m->nm * al.ld));
+  chk(dalloc(&b->d_acolelpr, (size_t)m->nm * al.ld));
+  if (e == hipSuccess)
+    chk(hipMemcpy(al.d_block_region, br.data(), br.size() * 4, hipMemcpyHostToDevice));
+  if (e == hipSuccess)
+    chk(hipMemcpy(al.d_astart, al.astart.data(), al.astart.size() * 8, hipMemcpyHostToDevice));
+  if (e == hipSuccess) chk(launch_align_codes(b, c->stream));
+  if (e == hipSuccess) chk(hipStreamSynchronize(c->stream));
+  if (e != hipSuccess) {
+    // leave the batch as before the call: the next per-region call retries
+    dfree(al.d_codes);
+    dfree(al.d_block_region);
+    dfree(al.d_astart);
+    dfree(b->d_alogz);
+    dfree(b->d_acolelpr);
+    al.astart.clear();
+    al.block0.clear();
+    al.ld = 0;
+  }
+  HIP_TRY(c, e);
   return PCSF_OK;
 }
 
@@ -257,7 +307,7 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
       return set_err(c, PCSF_EINVAL, "FP64 kernel: tree workspace exceeds 64 KiB of LDS");
     HIP_TRY(c, e);
     if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[1], c->stream));
-    HIP_TRY(c, launch_region_reduce_aligned(b, ch, c->stream));
+    HIP_TRY(c, launch_region_reduce_aligned(b, ch, b->d_loglik, b->d_elpr, c->stream));
     if (c->timing) {
       HIP_TRY(c, hipEventRecord(b->ev[2], c->stream));
       HIP_TRY(c, hipEventSynchronize(b->ev[2]));
@@ -272,8 +322,8 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
 }
 
 // Gradient pass at tree scales rho (nreg * nm) over the pairs in `active`
-// (optional): per-region lpr into d_loglik and d lpr / d rho into d_elpr
-// (the backward reuses the elpr buffers).  tangent = false: the FP64
+// (optional): per-region lpr into d_gloglik and d lpr / d rho into d_grad
+// (the per-column buffers of the forward hold the column values).  tangent = false: the FP64
 // outside-pass adjoint, which also accumulates, if d_gbr is given, the
 // per-region sums of d ln z / d t_br (nreg * nm * nbr); tangent = true: the
 // FP32 MFMA tangent pass (d lpr / d rho only).
@@ -324,7 +374,7 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
     if (e == hipErrorInvalidValue && !tangent)
       return set_err(c, PCSF_EINVAL, "adjoint kernel: tree workspace exceeds 64 KiB of LDS");
     HIP_TRY(c, e);
-    HIP_TRY(c, launch_region_reduce_aligned(b, ch, c->stream));
+    HIP_TRY(c, launch_region_reduce_aligned(b, ch, b->d_gloglik, b->d_grad, c->stream));
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return PCSF_OK;
@@ -455,7 +505,7 @@ int pcsf_newick_tree(const char* text, const char* keep_csv, int max_nodes, int*
 
 int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double* branch,
                       int nm, const double* S, const double* pi, pcsf_model** out) {
-  if (!c || !out) return PCSF_EINVAL;
+  if (!out) return PCSF_EINVAL;
   *out = nullptr;
   if (n < 3 || n % 2 == 0 || !children || !branch || nm < 1 || nm > 8 || !S || !pi)
     return set_err(c, PCSF_EINVAL, "CamlPaml.PhyloModel.make: bad tree or model count");
@@ -487,6 +537,23 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   for (double bl : m->branch)
     if (!(bl >= 0.0) || !std::isfinite(bl))
       return set_err(c, PCSF_EINVAL, "CamlPaml.PhyloModel.make: negative or missing branch");
+  // The eigensystems are those of reversible models (ratematrix.cpp), which
+  // an ECM's symmetric exchangeabilities give; a non-symmetric S would need
+  // the reference's complex path (lib/CamlPaml/Q.ml:124-141,218-223), which
+  // this build does not carry, so it is refused rather than symmetrised.
+  for (int q = 0; q < nm; ++q)
+    for (int i = 0; i < 64; ++i)
+      for (int j = 0; j < i; ++j) {
+        const double a = S[q * 4096 + i * 64 + j], b = S[q * 4096 + j * 64 + i];
+        if (!(std::fabs(a - b) <= 1e-12 * std::max(std::fabs(a), std::fabs(b)))) {
+          char buf[192];
+          std::snprintf(buf, sizeof buf,
+                        "CamlPaml.PhyloModel.make: model %d is not reversible (S[%d,%d] = %g, "
+                        "S[%d,%d] = %g); only symmetric exchangeabilities are supported",
+                        q, i, j, a, j, i, b);
+          return set_err(c, PCSF_EINVAL, buf);
+        }
+      }
   m->eig.resize(nm);
   m->pi.assign(pi, pi + 64 * nm);
   std::vector<double> V(nm * 4096), Vi(nm * 4096), lam(nm * 64), lnpi(nm * 64), QT(nm * 4096);
@@ -523,6 +590,10 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
     for (int& v : m->mfma_variant) v = 2;
   if (m->slots > kMaxSlots)
     return set_err(c, PCSF_EINVAL, "tree needs more than 6 live partial tiles");
+  if (!c) {  // host-only model: simulation and model facts, no device work
+    *out = guard.release();
+    return PCSF_OK;
+  }
 
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, dalloc(&m->d_VT, V.size()));
@@ -556,8 +627,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   m->n_lsl_t = (int)lsl_t.size();
   HIP_TRY(c, dalloc(&m->d_lsl, std::max<size_t>(1, lsl.size())));
   HIP_TRY(c, dalloc(&m->d_lsl_t, std::max<size_t>(1, lsl_t.size())));
-  HIP_TRY(c, dalloc(&m->d_perr, (size_t)8));
-  HIP_TRY(c, dalloc(&m->d_perrval, (size_t)1));
+  HIP_TRY(c, dalloc(&m->d_perr, (size_t)1));
   HIP_TRY(c, alloc_psets(m->fixed, nm, m->nbr, true, (int)m->cherries.size() / 3));
   std::vector<int32_t> lc32(m->lc.begin(), m->lc.end()), rc32(m->rc.begin(), m->rc.end());
   auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -599,7 +669,6 @@ void pcsf_model_destroy(pcsf_model* m) {
   dfree(m->d_lsl);
   dfree(m->d_lsl_t);
   dfree(m->d_perr);
-  dfree(m->d_perrval);
   free_psets(m->fixed);
   delete m;
 }
@@ -614,7 +683,7 @@ int pcsf_model_info(const pcsf_model* m, int* n, int* nl, int* nm, int* slots) {
 }
 
 int pcsf_model_transition(pcsf_model* m, int model, double rho, double* P_out) {
-  if (!m || !P_out) return PCSF_EINVAL;
+  if (!m || !P_out || !m->ctx) return PCSF_EINVAL;
   pcsf_ctx* c = m->ctx;
   if (model < 0 || model >= m->nm) return set_err(c, PCSF_EINVAL, "model index out of range");
   if (!rho_valid(rho))
@@ -634,7 +703,7 @@ int pcsf_model_transition(pcsf_model* m, int model, double rho, double* P_out) {
 
 int pcsf_batch_create(pcsf_model* m, const uint8_t* codes, int64_t ncols,
                       const int64_t* off, int32_t nreg, pcsf_batch** out) {
-  if (!m || !out) return PCSF_EINVAL;
+  if (!m || !out || !m->ctx) return PCSF_EINVAL;
   pcsf_ctx* c = m->ctx;
   *out = nullptr;
   if (ncols < 0 || nreg < 0 || (ncols > 0 && !codes) || !off)
@@ -659,16 +728,30 @@ int pcsf_batch_create(pcsf_model* m, const uint8_t* codes, int64_t ncols,
   HIP_TRY(c, dalloc(&b->d_colelpr, (size_t)m->nm * b->ld));
   HIP_TRY(c, dalloc(&b->d_loglik, std::max<size_t>(1, (size_t)nreg * m->nm)));
   HIP_TRY(c, dalloc(&b->d_elpr, std::max<size_t>(1, (size_t)nreg * m->nm)));
+  HIP_TRY(c, dalloc(&b->d_gloglik, std::max<size_t>(1, (size_t)nreg * m->nm)));
+  HIP_TRY(c, dalloc(&b->d_grad, std::max<size_t>(1, (size_t)nreg * m->nm)));
   HIP_TRY(c, hipMemcpy(b->d_offsets, off, sizeof(int64_t) * (nreg + 1), hipMemcpyHostToDevice));
-  // upload in chunks through a staging buffer, then make leaf-major
+  // upload through a staging buffer (with a flag word for invalid codes at
+  // its end), then make leaf-major
   uint8_t* stage = nullptr;
   const size_t bytes = (size_t)ncols * m->nl;
-  HIP_TRY(c, dalloc(&stage, std::max<size_t>(bytes, 1)));
-  if (bytes) HIP_TRY(c, hipMemcpy(stage, codes, bytes, hipMemcpyHostToDevice));
-  hipError_t e = launch_transpose_codes(stage, ncols, m->nl, b->d_codes, b->ld, c->stream);
+  const size_t flag_at = (bytes + 3) / 4 * 4;
+  HIP_TRY(c, dalloc(&stage, flag_at + 4));
+  int* d_bad = reinterpret_cast<int*>(stage + flag_at);
+  int bad = 0;
+  hipError_t e = hipMemsetAsync(d_bad, 0, 4, c->stream);
+  if (e == hipSuccess && bytes) e = hipMemcpyAsync(stage, codes, bytes, hipMemcpyHostToDevice,
+                                                    c->stream);
+  if (e == hipSuccess)
+    e = launch_transpose_codes(stage, ncols, m->nl, b->d_codes, b->ld, d_bad, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(stage);
   HIP_TRY(c, e);
+  if (bad)
+    return set_err(c, PCSF_EINVAL,
+                   "pcsf_batch_create: a leaf code is neither a codon index (0..63) nor "
+                   "PCSF_MARGINALIZE (255)");
   for (auto& ev : b->ev) HIP_TRY(c, hipEventCreate(&ev));
   *out = guard.release();
   return PCSF_OK;
@@ -676,7 +759,7 @@ int pcsf_batch_create(pcsf_model* m, const uint8_t* codes, int64_t ncols,
 
 int pcsf_batch_create_frames(pcsf_model* m, const uint8_t* nuc, int64_t n_nuc,
                              const int64_t* roff, int32_t nreg, int frames, pcsf_batch** out) {
-  if (!m || !out) return PCSF_EINVAL;
+  if (!m || !out || !m->ctx) return PCSF_EINVAL;
   pcsf_ctx* c = m->ctx;
   *out = nullptr;
   if (n_nuc < 0 || nreg < 0 || (n_nuc > 0 && !nuc) || !roff ||
@@ -713,6 +796,8 @@ int pcsf_batch_create_frames(pcsf_model* m, const uint8_t* nuc, int64_t n_nuc,
   HIP_TRY(c, dalloc(&b->d_colelpr, (size_t)m->nm * b->ld));
   HIP_TRY(c, dalloc(&b->d_loglik, std::max<size_t>(1, (size_t)nbr * m->nm)));
   HIP_TRY(c, dalloc(&b->d_elpr, std::max<size_t>(1, (size_t)nbr * m->nm)));
+  HIP_TRY(c, dalloc(&b->d_gloglik, std::max<size_t>(1, (size_t)nbr * m->nm)));
+  HIP_TRY(c, dalloc(&b->d_grad, std::max<size_t>(1, (size_t)nbr * m->nm)));
   HIP_TRY(c, hipMemcpy(b->d_offsets, foff.data(), sizeof(int64_t) * foff.size(),
                        hipMemcpyHostToDevice));
   uint8_t* d_nuc = nullptr;
@@ -751,6 +836,8 @@ void pcsf_batch_destroy(pcsf_batch* b) {
   dfree(b->al.d_astart);
   dfree(b->d_alogz);
   dfree(b->d_acolelpr);
+  dfree(b->d_gloglik);
+  dfree(b->d_grad);
   free_psets(b->rs);
   for (auto& ev : b->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -758,6 +845,23 @@ void pcsf_batch_destroy(pcsf_batch* b) {
 }
 
 int64_t pcsf_batch_columns(const pcsf_batch* b) { return b ? b->ncols : -1; }
+
+int pcsf_batch_codes(pcsf_batch* b, uint8_t* codes, int64_t* region_offsets) {
+  if (!b) return PCSF_EINVAL;
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  if (region_offsets) std::copy(b->offsets.begin(), b->offsets.end(), region_offsets);
+  if (!codes || b->ncols == 0) return PCSF_OK;
+  HIP_TRY(c, hipSetDevice(c->device));
+  // leaf-major rows of ld bytes on the device -> n_cols x nl on the host
+  std::vector<uint8_t> rows((size_t)m->nl * b->ncols);
+  HIP_TRY(c, hipMemcpy2D(rows.data(), b->ncols, b->d_codes, b->ld, b->ncols, m->nl,
+                         hipMemcpyDeviceToHost));
+  for (int l = 0; l < m->nl; ++l)
+    for (int64_t k = 0; k < b->ncols; ++k)
+      codes[(size_t)k * m->nl + l] = rows[(size_t)l * b->ncols + k];
+  return PCSF_OK;
+}
 
 int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int precision,
                  double* loglik, double* elpr, int save_for_backward) {
@@ -1126,8 +1230,8 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
     // FP64: the outside-pass adjoint; MFMA precisions: the FP32 tangent pass
     int rc = adjoint_regions(b, rho.data(), act.data(), nullptr, precision != PCSF_FP64);
     if (rc) return rc;
-    HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(D.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(L.data(), b->d_gloglik, np * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(D.data(), b->d_grad, np * 8, hipMemcpyDeviceToHost));
     return PCSF_OK;
   };
   for (int it = 0; it < 100; ++it) {
@@ -1290,7 +1394,7 @@ int pcsf_backward(pcsf_batch* b, const double* grad_loglik, double* grad_rho,
   int rc = adjoint_regions(b, b->last_rho.data(), nullptr, d_gbr, tangent);
   std::vector<double> dl(np), gb;
   if (!rc && np) {
-    hipError_t e = hipMemcpy(dl.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpy(dl.data(), b->d_grad, np * 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess && d_gbr) {
       gb.resize(ngb);
       e = hipMemcpy(gb.data(), d_gbr, ngb * 8, hipMemcpyDeviceToHost);

@@ -12,7 +12,9 @@
 // still written alignment by alignment, in input order.
 //
 // Extra options (no counterpart in the reference):
-//   --precision=fp64|fp32|fp16s   arithmetic of the pruning (default fp32)
+//   --precision=fp64|fp32|fp16s   arithmetic of the pruning (default fp64: the
+//                                 reference's digits; fp32 and split-fp16 agree
+//                                 with it to 1e-3 dB per region at fixed scales)
 //   --device=N                    GPU index (default 0)
 //   --batchBytes=N                leaf-code bytes packed per GPU call (default 1 GiB)
 // The omega strategy is not part of this build.
@@ -90,7 +92,7 @@ struct Options {
   bool all_scores = false;
   int procs = 1;
   bool bls = false, anc_comp = false, dna = false, aa = false, debug = false;
-  int precision = PCSF_FP32;
+  int precision = PCSF_FP64;
   int device = 0;
   long long batch_bytes = 1LL << 30;
   std::vector<std::string> positional;
@@ -123,7 +125,7 @@ const char* kUsage =
     "    --aa                       include amino acid translation in output\n"
     "    --debug                    print extra information about parameters and errors\n\n"
     "  GPU:\n"
-    "    --precision=fp64|fp32|fp16s  arithmetic of the pruning kernel (default fp32)\n"
+    "    --precision=fp64|fp32|fp16s  arithmetic of the pruning kernel (default fp64)\n"
     "    --device=N                 GPU index (default 0)\n"
     "    --batchBytes=N             leaf-code bytes packed per GPU call (default 1 GiB)\n";
 

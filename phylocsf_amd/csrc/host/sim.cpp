@@ -97,6 +97,8 @@ static int simulate(pcsf_model* m, int32_t nreg, const int64_t* off, const int32
     return PCSF_EINVAL;
   if (gap_p < 0 || gap_p >= 1 || n_p < 0 || n_p > 1 || (gap_mode == 1 && !(gap_run >= 1)))
     return PCSF_EINVAL;
+  // runs of mean gap_run can cover at most gap_run / (gap_run + 1)
+  if (gap_mode == 1 && gap_p / (1 - gap_p) > gap_run) return PCSF_EINVAL;
   for (int32_t r = 0; r < nreg; ++r)
     if (region_model[r] < 0 || region_model[r] >= m->nm || !(region_rho[r] > 0) ||
         off[r + 1] < off[r])
@@ -142,10 +144,14 @@ static int simulate(pcsf_model* m, int32_t nreg, const int64_t* off, const int32
           cached_q = q;
           cached_rho = rho;
         }
-        // gap-run state per species for this region (gap_mode 1)
+        // gap-run state per species for this region (gap_mode 1): a two-state
+        // chain that leaves a run with probability 1 / gap_run per column
+        // (geometric run lengths of mean gap_run) and enters one with the
+        // probability that makes its stationary gap fraction gap_p:
+        // p_start / (p_start + p_stop) = gap_p
         std::vector<int> in_gap(nl, 0);
-        const double p_start = (gap_mode == 1) ? gap_p / (gap_run * (1 - gap_p) + gap_p) : 0;
         const double p_stop = (gap_mode == 1) ? 1.0 / gap_run : 0;
+        const double p_start = (gap_mode == 1) ? gap_p * p_stop / (1 - gap_p) : 0;
         for (int64_t c = off[r]; c < off[r + 1]; ++c) {
           Stream st(&ph, (uint32_t)r, (uint64_t)c);
           state[n - 1] = draw(root.data(), 64, st.uniform());

@@ -104,8 +104,9 @@ struct pcsf_model {
   std::vector<double> cached_rho;
   int cached_fmt = -1;
   PSets fixed;
-  int* d_perr = nullptr;     // error record of the P builder
-  double* d_perrval = nullptr;
+  // failure record of the P builder: the minimum packed (set, branch, row,
+  // column, check) key over failing rows, all ones when none failed
+  unsigned long long* d_perr = nullptr;
 };
 
 // Region-aligned copy of a batch's codes: every region starts on a multiple
@@ -139,6 +140,10 @@ struct pcsf_batch {
   PSets rs;                         // per-region transition sets (one chunk)
   double* d_alogz = nullptr;        // nm * al.ld, per aligned column
   double* d_acolelpr = nullptr;
+  // gradient passes (pcsf_backward, mle_grad) reduce into these, so that the
+  // last forward's d_loglik / d_elpr stay as they were
+  double* d_gloglik = nullptr;      // nreg * nm: lpr at the differentiated scales
+  double* d_grad = nullptr;         // nreg * nm: d lpr / d rho
 };
 
 // Per-region-scale launch description (one chunk of regions).
@@ -149,8 +154,10 @@ struct RegionChunk {
 };
 
 // ---- kernel launchers (kernels/*.hip) --------------------------------------
+// bad (may be null) gets a nonzero value if a code is neither a codon index
+// (0..63) nor PCSF_MARGINALIZE
 hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uint8_t* dst,
-                                  int64_t ld, hipStream_t s);
+                                  int64_t ld, int* bad, hipStream_t s);
 hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int fmt,
                          hipStream_t s);
 hipError_t launch_cherry_tables(const pcsf_model* m, const PSets& sets, int nsets,
@@ -168,8 +175,9 @@ hipError_t launch_adjoint_fp64(const pcsf_model* m, const pcsf_batch* b, const R
                                double* grad_branch, hipStream_t s);
 hipError_t launch_prune_tangent_regions(const pcsf_model* m, const pcsf_batch* b,
                                         const RegionChunk& ch, hipStream_t s);
+// sums over each region of the chunk into loglik / elpr (nreg * nm)
 hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& ch,
-                                        hipStream_t s);
+                                        double* loglik, double* elpr, hipStream_t s);
 // Launch shapes of the fused kernel that the autotuner chooses among.
 struct MfmaVariant {
   int wpg;        // waves per workgroup (16 columns each) sharing one LDS P tile

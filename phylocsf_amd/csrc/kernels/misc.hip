@@ -13,16 +13,23 @@
 namespace {
 
 __global__ void transpose_codes_kernel(const uint8_t* __restrict__ src, int64_t ncols, int nl,
-                                       uint8_t* __restrict__ dst, int64_t ld) {
+                                       uint8_t* __restrict__ dst, int64_t ld,
+                                       int* __restrict__ bad) {
   // 64 columns x nl leaves per workgroup, staged through LDS
   __shared__ uint8_t tile[64 * 257];
   const int64_t c0 = (int64_t)blockIdx.x * 64;
+  bool invalid = false;
   for (int l0 = 0; l0 < nl; l0 += 256) {
     const int lw = min(256, nl - l0);
     for (int e = threadIdx.x; e < 64 * lw; e += blockDim.x) {
       const int c = e / lw, l = e - c * lw;
       const int64_t col = c0 + c;
-      tile[c * 257 + l] = (col < ncols) ? src[col * nl + l0 + l] : (uint8_t)PCSF_MARGINALIZE;
+      const uint8_t x = (col < ncols) ? src[col * nl + l0 + l] : (uint8_t)PCSF_MARGINALIZE;
+      // a code that is neither a codon nor `Marginalize would index past the
+      // 65-row leaf tables: it is stored as `Marginalize and reported
+      const bool ok = x < 64 || x == PCSF_MARGINALIZE;
+      invalid = invalid || !ok;
+      tile[c * 257 + l] = ok ? x : (uint8_t)PCSF_MARGINALIZE;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < 64 * lw; e += blockDim.x) {
@@ -32,6 +39,7 @@ __global__ void transpose_codes_kernel(const uint8_t* __restrict__ src, int64_t 
     }
     __syncthreads();
   }
+  if (bad && __any(invalid) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
 }
 
 __global__ __launch_bounds__(256) void region_reduce_kernel(
@@ -124,21 +132,21 @@ hipError_t launch_align_codes(const pcsf_batch* b, hipStream_t s) {
 }
 
 hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& ch,
-                                        hipStream_t s) {
+                                        double* loglik, double* elpr, hipStream_t s) {
   if (ch.nreg == 0) return hipSuccess;
   dim3 grid(ch.nreg, b->m->nm);
   hipLaunchKernelGGL(region_reduce_aligned_kernel, grid, dim3(256), 0, s, b->d_alogz,
                      b->d_acolelpr, b->d_offsets, b->al.d_astart, ch.sets->d_active, ch.r0,
-                     b->m->nm, b->al.ld, b->d_loglik, b->d_elpr);
+                     b->m->nm, b->al.ld, loglik, elpr);
   return hipGetLastError();
 }
 
 hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uint8_t* dst,
-                                  int64_t ld, hipStream_t s) {
+                                  int64_t ld, int* bad, hipStream_t s) {
   const int64_t blocks = (ld + 63) / 64;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(transpose_codes_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src,
-                     ncols, nl, dst, ld);
+                     ncols, nl, dst, ld, bad);
   return hipGetLastError();
 }
 

@@ -5,8 +5,12 @@
 // branches of one set.  Reproduces Q.Diag.real_to_Pt
 // (lib/CamlPaml/Q.ml:211-249): entries below zero are an error when
 // |x| > 1e-6 and are set to 0 otherwise; the row sum must be within 1e-6 of
-// 1; the diagonal is replaced by 1 - sum(off-diagonal).  The first failure
-// is recorded in perr = {code, branch, model, row, col} for the host.
+// 1; the diagonal is replaced by 1 - sum(off-diagonal).  The failure the
+// reference raises first -- lowest set, then branch, then row, then column,
+// with a row's negative entries before its sum and diagonal checks -- is the
+// minimum of a packed key over every failing (set, branch, row), taken with
+// one 64-bit atomicMin, so the record does not depend on which workgroup ran
+// first (fail_key below; all ones when nothing failed).
 //
 // Outputs, all [set][branch][...]; a leaf branch (br < nl) is read only
 // through its leaf messages and an internal one only through its tile, so
@@ -74,16 +78,28 @@ __device__ __forceinline__ double lanes_sum(double v) {
 // unscaled) and B = V^T sits in LDS for all of them; only e changes from
 // branch to branch.  B is stored with row k's 16-column halves swapped when
 // k is odd, so the two rows a half-wave reads per step fall in disjoint banks.
+// Failure key: ((set * nbr + br) * 64 + row) * 65 + col, col = 64 for the
+// row-sum and diagonal checks (made after the row's entries), shifted left by
+// 2 over the check: 0 t < 0 (raised before any row), 1 negative entry, 2 row
+// sum, 3 diagonal.  The host decodes it and recomputes the message.
+__device__ __forceinline__ unsigned long long fail_key(int set, int nbr, int br, int row, int col,
+                                                      int check) {
+  const unsigned long long pos =
+      (((unsigned long long)set * nbr + br) * 64 + row) * 65 + col;
+  return (pos << 2) | (unsigned long long)check;
+}
+
 template <bool WITH_D>
 __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     const double* __restrict__ VT, const double* __restrict__ Vinv,
     const double* __restrict__ lam, const double* __restrict__ branch,
     const double* __restrict__ rho, const uint8_t* __restrict__ active, int nm, int nbr,
     int nbw, int nl, double* __restrict__ P64T, float* __restrict__ PT32,
-    float* __restrict__ Pfrag, int fmt, double* __restrict__ lnPcc, int* __restrict__ perr,
-    double* __restrict__ perrval, float* __restrict__ DT32, float* __restrict__ Dfrag) {
+    float* __restrict__ Pfrag, int fmt, double* __restrict__ lnPcc,
+    unsigned long long* __restrict__ perr, float* __restrict__ DT32,
+    float* __restrict__ Dfrag) {
   __shared__ double Bs[64 * 64];  // V^T, swizzled
-  __shared__ double ptot[4][64], poff[4][64], pbad[4][64];  // per wave, per row of P
+  __shared__ double ptot[4][64], poff[4][64];  // per wave, per row of P
   __shared__ int pjbad[4][64];
   __shared__ double pdiag[64], dinv[64];  // fixed-up diagonal and its inverse
   const int set = blockIdx.y, q = set % nm, tid = threadIdx.x;
@@ -103,11 +119,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     const double* A = Vinv + (size_t)q * 4096 + 16 * w + lr;
     const double bl = branch[br];
     const double t = rho[set] * bl;
-    if (t < 0.0) {
-      if (tid == 0 && atomicCAS(perr, 0, 4) == 0) {
-        perr[1] = br; perr[2] = set; perr[3] = 0; perr[4] = 0; *perrval = t;
-      }
-    }
+    if (t < 0.0 && tid == 0) atomicMin(perr, fail_key(set, nbr, br, 0, 0, 0));
     f64x4 acc[4] = {};
     f64x4 dacc[WITH_D ? 4 : 1] = {};
     {
@@ -147,26 +159,17 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
 #pragma unroll
         for (int r = 0; r < 4; ++r) mn = fmin(mn, acc[ct][r]);
       if (__any(mn < -kTol)) {
-        double bv[4];
         int jb[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
-          bv[ct] = 0.0;
           jb[ct] = 64;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const double x = acc[ct][r];
-            if (x < -kTol && jb[ct] == 64) { jb[ct] = 16 * w + 4 * r + lk; bv[ct] = x; }
-          }
+          for (int r = 0; r < 4; ++r)
+            if (acc[ct][r] < -kTol && jb[ct] == 64) jb[ct] = 16 * w + 4 * r + lk;
 #pragma unroll
-          for (int m = 16; m <= 32; m <<= 1) {
-            const int j2 = __shfl_xor(jb[ct], m);
-            const double b2 = __shfl_xor(bv[ct], m);
-            if (j2 < jb[ct]) { jb[ct] = j2; bv[ct] = b2; }
-          }
+          for (int m = 16; m <= 32; m <<= 1) jb[ct] = min(jb[ct], __shfl_xor(jb[ct], m));
         }
         // lane (lr, lk) stores row a = 16 lk + lr = lane, i.e. tile ct = lk
-        pbad[w][lane] = lk == 0 ? bv[0] : lk == 1 ? bv[1] : lk == 2 ? bv[2] : bv[3];
         pjbad[w][lane] = lk == 0 ? jb[0] : lk == 1 ? jb[1] : lk == 2 ? jb[2] : jb[3];
       } else {
         pjbad[w][lane] = 64;
@@ -199,20 +202,18 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     __syncthreads();
     if (tid < 64) {
       const int i = tid;
-      double tot = 0.0, off = 0.0, bad = 0.0;
-      int code = 0, jbad = 0;
+      double tot = 0.0, off = 0.0;
+      int code = 0, jbad = 64;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         tot += ptot[v][i];
         off += poff[v][i];
-        if (code == 0 && pjbad[v][i] < 64) { code = 1; jbad = pjbad[v][i]; bad = pbad[v][i]; }
+        if (code == 0 && pjbad[v][i] < 64) { code = 1; jbad = pjbad[v][i]; }
       }
       const double diag = 1.0 - off;
-      if (code == 0 && fabs(tot - 1.0) > kTol) { code = 2; bad = tot; }
-      if (code == 0 && !(diag <= 1.0 && diag > 0.0)) { code = 3; bad = diag; }
-      if (code != 0 && atomicCAS(perr, 0, code) == 0) {
-        perr[1] = br; perr[2] = set; perr[3] = i; perr[4] = jbad; *perrval = bad;
-      }
+      if (code == 0 && fabs(tot - 1.0) > kTol) code = 2;
+      if (code == 0 && !(diag <= 1.0 && diag > 0.0)) code = 3;
+      if (code != 0) atomicMin(perr, fail_key(set, nbr, br, i, jbad, code));
       pdiag[i] = diag;
       dinv[i] = 1.0 / diag;
     }
@@ -384,7 +385,7 @@ hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int 
   hipLaunchKernelGGL(k, grid, dim3(256), 0, s, m->d_VT, m->d_Vinv, m->d_lam,
                      m->d_branch, sets.d_rho, sets.d_active, m->nm, m->nbr, nbw, m->nl,
                      sets.fp64 ? sets.d_P64T : nullptr, sets.d_PT32, sets.d_Pfrag, fmt,
-                     sets.d_lnPcc, m->d_perr, m->d_perrval, sets.d_DT32,
+                     sets.d_lnPcc, m->d_perr, sets.d_DT32,
                      sets.d_Dfrag);
   return hipGetLastError();
 }

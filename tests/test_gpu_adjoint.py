@@ -177,3 +177,26 @@ def test_mle_grad_optimality(ctx, params_dir):
             assert abs(gg["rho"][r, q] - gb["rho"][r, q]) <= 0.01 * gb["rho"][r, q] + 1e-9
     g32 = P.score(b, "mle_grad", P.PCSF_FP32)
     assert np.all(np.abs(g32["score"] - gg["score"]) <= 1e-3)
+
+
+def test_device_results_survive_backward(ctx, mammals):
+    """pcsf_batch_device_results keeps the last forward's values through a
+    backward (the gradient passes reduce into buffers of their own)."""
+    import ctypes as C
+    from phylocsf_amd import _lib
+    _say("device results after backward")
+    m, codes, off, rho = mammals
+    b = P.Batch(m, codes, off)
+    # the HIP runtime this process already loaded (through the library)
+    path = next(ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64" in ln)
+    hip = C.CDLL(path)
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    for prec in (P.PCSF_FP64, P.PCSF_FP32):
+        ll, el = b.forward(rho, prec, per_region=True)
+        b.backward(np.ones_like(rho))
+        dl, de = C.c_void_p(), C.c_void_p()
+        assert _lib.lib().pcsf_batch_device_results(b.h, C.byref(dl), C.byref(de)) == 0
+        got_l, got_e = np.zeros_like(ll), np.zeros_like(el)
+        assert hip.hipMemcpy(got_l.ctypes.data, dl, got_l.nbytes, 2) == 0
+        assert hip.hipMemcpy(got_e.ctypes.data, de, got_e.nbytes, 2) == 0
+        assert np.array_equal(got_l, ll) and np.array_equal(got_e, el), prec

@@ -4,8 +4,8 @@ Every reading frame of packed nucleotide regions is read on the GPU; the
 codes must equal the oracle's restatement of PhyloCSF's candidate regions
 and pleaves (src/PhyloCSF.ml:198-246; oracle/cli_oracle.py) for frames 1, 3
 and 6, including regions whose length is not a multiple of 3, shorter than
-a codon, or empty.  Equality of codes is checked through the FP64 forward,
-which is deterministic: the same codes give bitwise the same log-likelihoods.
+a codon, or empty.  The codes are read back from the device and compared
+byte for byte, and the FP64 forwards of both batches agree bitwise.
 """
 import numpy as np
 import pytest
@@ -69,6 +69,10 @@ def test_frames_match_oracle_pleaves(ctx):
         bh = P.Batch.from_regions(m, regions)
         assert bg.n_regions == bh.n_regions == len(regions)
         assert bg.n_cols == bh.n_cols
+        # the GPU-built codes, byte for byte
+        codes, coff = bg.codes()
+        assert np.array_equal(codes, np.concatenate(regions, axis=0))
+        assert np.array_equal(coff, np.cumsum([0] + [r.shape[0] for r in regions]))
         lg, eg = bg.forward(1.0, P.PCSF_FP64)
         lh, eh = bh.forward(1.0, P.PCSF_FP64)
         assert np.array_equal(lg, lh) and np.array_equal(eg, eh)
@@ -85,6 +89,7 @@ def test_frame_one_equals_simulated_codes(ctx):
     assert nuc.shape == (m.tree.nl, 3 * codes.shape[0])
     bg = P.Batch.frames(m, nuc, noff, 1)
     bh = P.Batch(m, codes, coff)
+    assert np.array_equal(bg.codes()[0], codes) and np.array_equal(bh.codes()[0], codes)
     assert np.array_equal(bg.forward(1.0, P.PCSF_FP64)[0], bh.forward(1.0, P.PCSF_FP64)[0])
     # the reference row is never masked
     assert set(np.unique(nuc[0])) <= set(b"ACGT")

@@ -9,8 +9,9 @@
   (a near-tie at Brent's last sqrt(eps)-sized step can flip on 1e-12-level
   differences), so rho agrees to 1e-6 and the score to 1e-6 dB.  With FP32 likelihoods
   the Brent comparisons can take another path near the flat optimum; those
-  scores are held to the reference's published 0.01-wide windows and to
-  5e-3 dB of the FP64 mle.
+  scores are held to the reference's published 0.01-wide windows and to the
+  bound its 1% stopping rule allows around the FP64 run (tests/mle_bounds.py)
+  plus 1e-3 dB.
 """
 import os
 
@@ -20,6 +21,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 import phylocsf_amd as P  # noqa: E402
+from mle_bounds import brent_stop_bound_db  # noqa: E402
 from oracle import cli_oracle as CO  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
@@ -82,7 +84,14 @@ def test_mle_matches_oracle_maximizer(ctx, params_dir):
         assert abs(g64["rho"][r, 1] - ref["rho_n"]) <= 1e-6 * ref["rho_n"], r
         assert abs(g64["score"][r] - ref["score"]) <= 1e-6, r
         assert abs(g64["anc_comp"][r] - ref["anc_comp"]) <= 1e-5, r
-        assert abs(g32["score"][r] - g64["score"][r]) <= 5e-3, r
+    # FP32 likelihoods may steer Brent along another path: held to the 1%
+    # stopping rule's own bound around the FP64 run, plus the 1e-3 dB rounding
+    # contract of one evaluation
+    bound = brent_stop_bound_db(b, g64["rho"], g32["rho"]) + 1e-3
+    d = np.abs(g32["score"] - g64["score"])
+    _say("  FP32 mle vs FP64 mle: max |d score| %.3g dB, max d/bound %.3g"
+         % (d.max(), np.max(d / bound)))
+    assert np.all(d <= bound)
 
 
 def _regions_from_mfa(m, text, frames=1, orf="AsIs", remove_ref_gaps=False, min_codons=25):
@@ -153,3 +162,33 @@ def test_forward_per_region_many_sets(ctx, params_dir):
                 ref_l, ref_e = om.lpr(q, rho[r, q], codes[off[r]:off[r + 1]])
                 assert abs(ll[r, q] - ref_l) <= tol * max(1.0, abs(ref_l)), (prec, r, q)
                 assert abs(el[r, q] - ref_e) <= 10 * tol * max(1.0, abs(ref_e)), (prec, r, q)
+
+
+def test_mle_find_init_retry_path_matches_oracle(ctx, params_dir):
+    """Regions whose likelihood at rho = 1 loses to an end of [0.01, 10] send
+    Fit.find_init into its retry loop (lib/CamlPaml/Fit.ml:35-47), drawing
+    from OCaml's Random.init 0 stream: the library's restatement of that
+    stream (host/maximize.cpp) and the oracle's must give the same retries,
+    hence the same starting point for Brent and the same estimates (a
+    different start ends elsewhere inside Brent's 1% interval)."""
+    _say("mle retry path vs oracle")
+    m = P.PhyloCSFModel.from_paramset(ctx, "12flies")
+    lengths = np.array([60, 80, 40, 70, 50, 90])
+    true_rho = np.array([0.012, 0.02, 0.03, 0.05, 9.0, 12.0])
+    codes, off = m.simulate(lengths, np.array([0, 1, 0, 1, 0, 1]), true_rho, seed=77)
+    om = _oracle_model(params_dir, "12flies")
+    b = P.Batch(m, codes, off)
+    g = P.score(b, "mle", P.PCSF_FP64)
+    retried = 0
+    for r in range(len(lengths)):
+        ref = om.score_mle(codes[off[r]:off[r + 1]])
+        assert g["status"][r] == 0
+        assert abs(g["rho"][r, 0] - ref["rho_c"]) <= 1e-6 * ref["rho_c"], r
+        assert abs(g["rho"][r, 1] - ref["rho_n"]) <= 1e-6 * ref["rho_n"], r
+        assert abs(g["score"][r] - ref["score"]) <= 1e-6, r
+        for q in range(2):
+            l1, _ = om.lpr(q, 1.0, codes[off[r]:off[r + 1]])
+            llo, _ = om.lpr(q, 0.01, codes[off[r]:off[r + 1]])
+            lhi, _ = om.lpr(q, 10.0, codes[off[r]:off[r + 1]])
+            retried += int(l1 <= llo or l1 <= lhi)
+    assert retried >= 6  # the retry loop ran for most (region, model) pairs

@@ -18,6 +18,7 @@ pytestmark = pytest.mark.gpu
 
 import phylocsf_amd as P  # noqa: E402
 from phylocsf_amd import workloads as W  # noqa: E402
+from mle_bounds import brent_stop_bound_db  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 TOL_DB = 1e-3
@@ -124,7 +125,15 @@ def test_fp16s_mle(ctx):
     g64 = P.score(b, "mle", P.PCSF_FP64)
     g16 = P.score(b, "mle", P.PCSF_FP16S)
     assert (g16["status"] == 0).all()
-    assert np.all(np.abs(g16["score"] - g64["score"]) <= 5e-3)
+    bound = brent_stop_bound_db(b, g64["rho"], g16["rho"]) + TOL_DB
+    d = np.abs(g16["score"] - g64["score"])
+    _say("  max |d score| %.3g dB, max d/bound %.3g" % (d.max(), np.max(d / bound)))
+    assert np.all(d <= bound)
+    # the gradient maximiser: 1e-3 dB of its own FP64 run
+    q64 = P.score(b, "mle_grad", P.PCSF_FP64)
+    q16 = P.score(b, "mle_grad", P.PCSF_FP16S)
+    assert (q16["status"] == 0).all()
+    assert np.all(np.abs(q16["score"] - q64["score"]) <= TOL_DB)
 
 
 def test_edge_columns_fp16s(ctx):

@@ -75,3 +75,17 @@ def test_ecm_errors(tmp_path, params_dir):
         P.import_parameters(str(p))
     with pytest.raises(P.Failure):
         P.import_parameters(str(tmp_path / "missing.ECM"))
+
+
+def test_batch_rejects_codes_outside_the_alphabet():
+    """A leaf code is a codon index 0..63 or PCSF_MARGINALIZE (255); the Python
+    Batch refuses anything else before the library is called (the library
+    checks the same on the device, tests/test_gpu_errors.py)."""
+    from types import SimpleNamespace
+    model = SimpleNamespace(tree=SimpleNamespace(nl=4))
+    off = np.array([0, 3], np.int64)
+    for bad in (64, 100, 254):
+        codes = np.zeros((3, 4), np.uint8)
+        codes[1, 2] = bad
+        with pytest.raises(P.InvalidArgument):
+            P.Batch(model, codes, off)

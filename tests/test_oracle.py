@@ -35,14 +35,117 @@ def test_md5_rfc1321_vectors():
     assert O.md5(b"a" * 1000).hex() == __import__("hashlib").md5(b"a" * 1000).hexdigest()
 
 
+def test_md5_matches_hashlib_at_every_padding_length():
+    import hashlib
+    rng = np.random.default_rng(1)
+    for n in list(range(0, 130)) + [1000, 4097]:
+        msg = rng.integers(0, 256, n, dtype=np.uint8).tobytes()
+        assert O.md5(msg) == hashlib.md5(msg).digest(), n
+
+
 def test_ocaml_random_range_and_determinism():
-    # Stream values are parity-unpinned (no OCaml here); range and
-    # determinism are what can be checked.
     a = O.ocaml_random_floats(0, 6.9, 1000)
     b = O.ocaml_random_floats(0, 6.9, 1000)
     assert np.array_equal(a, b)
     assert (a >= 0).all() and (a < 6.9).all()
-    assert abs(a.mean() - 3.45) < 0.3
+    assert not np.array_equal(a, O.ocaml_random_floats(1, 6.9, 1000))
+
+
+
+
+def test_ocaml_random_recurrence():
+    """The stream obeys the generator's own recurrence, read from its outputs:
+    the 30-bit output n is (out[n-31] + (out[n-55] xor ((out[n-55] lsr 25)
+    land 31))) land (2^30 - 1) for n >= 55 (55 words of state, the new word
+    built from the ones written 31 and 55 outputs earlier).  Random.float
+    takes two outputs per draw, so output 2k is draw k's low part r1, whose
+    lags 31 and 55 are high parts r2 (known exactly): r1 is predicted and
+    the draw rebuilt from it in the same double arithmetic.  A wrong lag,
+    shift, mask or order of r1 and r2 in the float fails."""
+    n = 20000
+    f = O.ocaml_random_floats(0, 1.0, n)
+    x = f * 2.0 ** 30                 # r1 / 2^30 + r2, as rounded to a double
+    r2 = np.floor(x).astype(np.int64)  # output 2k + 1, exact
+    mask = (1 << 30) - 1
+    for k in range(28, n):
+        a = int(r2[k - 16])  # output 2k - 31
+        b = int(r2[k - 28])  # output 2k - 55
+        r1 = (a + (b ^ ((b >> 25) & 31))) & mask
+        assert (float(r1) / 2.0 ** 30 + float(r2[k])) / 2.0 ** 30 == f[k], k
+
+
+def test_ocaml_random_seeding_against_hashlib():
+    """Random.init seed = full_init [| seed |]: state word i starts as i, then
+    for i = 0 .. 109 the accumulator becomes MD5(accu ^ string_of_int seed)
+    (from accu = "x") and word i mod 55 is xor-ed with the digest's first four
+    bytes, little-endian, masked to 30 bits.  Restated here on hashlib's MD5
+    and the first outputs compared with the oracle's (C) restatement."""
+    import hashlib
+    for seed in (0, 1, 42, -7):
+        st = list(range(55))
+        accu = b"x"
+        for i in range(110):
+            accu = hashlib.md5(accu + str(seed).encode()).digest()
+            ex = accu[0] | (accu[1] << 8) | (accu[2] << 16) | (accu[3] << 24)
+            st[i % 55] = (st[i % 55] ^ ex) & 0x3FFFFFFF
+        idx = 0
+        outs = []
+        for _ in range(40):
+            idx = (idx + 1) % 55
+            cur = st[idx]
+            new = (st[(idx + 24) % 55] + (cur ^ ((cur >> 25) & 0x1F))) & 0x3FFFFFFF
+            st[idx] = new
+            outs.append(new)
+        f = O.ocaml_random_floats(seed, 1.0, 20)
+        exp = [(outs[2 * k] / 2.0 ** 30 + outs[2 * k + 1]) / 2.0 ** 30 for k in range(20)]
+        assert np.array_equal(f, np.array(exp)), seed
+
+
+def test_ocaml_random_uniformity():
+    """Random.float over 1e5 draws: chi-squared over 100 bins, Kolmogorov-
+    Smirnov, and no lag-one correlation."""
+    from scipy import stats
+    a = O.ocaml_random_floats(0, 1.0, 100000)
+    assert stats.chisquare(np.bincount((a * 100).astype(int), minlength=100)).pvalue > 1e-4
+    assert stats.kstest(a, "uniform").pvalue > 1e-4
+    r = np.corrcoef(a[1:], a[:-1])[0, 1]
+    assert abs(r) < 4 / np.sqrt(len(a))
+
+
+def test_find_init_retries_until_inside_beats_both_ends():
+    """Fit.find_init (lib/CamlPaml/Fit.ml:27-48) on a function whose ends beat
+    f(init): it retries at exp(ln lo + Random.float (ln hi - ln lo)) until a
+    point beats both ends; that point lies in [lo, hi], beats both ends, and
+    is the first retry draw that does (the draws come from Random.init 0)."""
+    import ctypes as C
+    L = O.lib()
+    FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+    L.o_find_init.argtypes = [FN, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                              C.c_double, C.POINTER(C.c_int)]
+    L.o_find_init.restype = C.c_double
+    lo, hi = 0.01, 10.0
+
+    def bump(x):  # maximum at 0.05; f(0.01) and f(10) both above f(1)
+        return -(np.log(x) - np.log(0.05)) ** 2 + (3.0 if x in (lo, hi) else 0.0) * 0
+
+    seen = []
+    cb = FN(lambda x, ctx: (seen.append(x), bump(x))[1])
+    ev = C.c_int(0)
+    x = L.o_find_init(cb, None, 250, 1, 1.0, lo, hi, C.byref(ev))
+    assert bump(1.0) <= bump(lo)  # init loses to an end: the retry loop runs
+    assert lo <= x <= hi and bump(x) > bump(lo) and bump(x) > bump(hi)
+    assert ev.value > 3 and ev.value == len(seen)
+    # the retries are the Random.init 0 stream mapped into [ln lo, ln hi]
+    draws = O.ocaml_random_floats(0, np.log(hi) - np.log(lo), ev.value - 3)
+    assert np.allclose(seen[3:], np.exp(np.log(lo) + draws), rtol=1e-15, atol=0)
+    first_ok = next(i for i, d in enumerate(np.exp(np.log(lo) + draws))
+                    if bump(d) > bump(lo) and bump(d) > bump(hi))
+    assert first_ok == ev.value - 4
+    # maxtries reached: the better end is taken
+    seen.clear()
+    x = L.o_find_init(FN(lambda x, ctx: 1.0 if x == hi else 0.0), None, 250, 1, 1.0, lo, hi,
+                      C.byref(ev))
+    assert x == hi and ev.value == 253
 
 
 def test_brent_finds_parabola_max():
