@@ -158,6 +158,29 @@ int pcsf_batch_codes(pcsf_batch *b, uint8_t *codes, int64_t *region_offsets);
 int pcsf_forward(pcsf_batch *b, const double *rho, int rho_per_region, int precision,
                  double *loglik, double *elpr, int save_for_backward);
 
+/* pcsf_forward (shared tree scales) over codes already in device memory,
+ * allocating nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The caller passes
+ *   d_codes           n_cols x n_leaves bytes on the device, the layout of
+ *                     pcsf_batch_create's codes (0..63 or PCSF_MARGINALIZE);
+ *   d_region_offsets  n_regions + 1 int64 on the device;
+ *   rho               n_models tree scales (host);
+ *   d_loglik, d_elpr  n_regions * n_models doubles on the device (outputs).
+ * Transition matrices are the model's own (rebuilt in place when rho or the
+ * precision changes).  Per-column work goes to one fixed device array of
+ * PCSF_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: a call
+ * needs about n_cols * (n_leaves + 16 * n_models) bytes of it (columns
+ * rounded up to 128) and is refused with PCSF_EINVAL beyond that (about
+ * 370,000 columns for 58 leaves).  Every call on one GPU shares that array,
+ * so calls that overlap on one GPU (other threads or streams) race.  Bad
+ * codes or offsets give PCSF_EINVAL.  Returns after the results are written
+ * (synchronous on the context's stream). */
+#define PCSF_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pcsf_forward_noalloc(pcsf_model *m, const uint8_t *d_codes, int64_t n_cols,
+                         const int64_t *d_region_offsets, int32_t n_regions, const double *rho,
+                         int precision, double *d_loglik, double *d_elpr);
+
 /* Backward of the last forward on this batch (same tree scales): given
  * dL/dloglik (n_regions*n_models), returns dL/drho (n_regions*n_models) and,
  * if grad_branch is non-NULL, dL/dt_br per region and model

@@ -23,6 +23,7 @@ PCSF_FP16S = 2
 PCSF_FP16 = 3
 PCSF_BF16 = 4
 PCSF_N_VARIANTS = 7  # launch shapes of the fused kernel (pcsf_autotune)
+PCSF_NOALLOC_SCRATCH_BYTES = 32 << 20  # scratch of pcsf_forward_noalloc, per GPU
 
 EXPORTED = [
     "pcsf_ctx_create", "pcsf_ctx_destroy", "pcsf_last_error", "pcsf_ctx_set_timing",
@@ -32,7 +33,7 @@ EXPORTED = [
     "pcsf_backward", "pcsf_batch_device_results", "pcsf_batch_last_timing",
     "pcsf_lik_columns", "pcsf_simulate", "pcsf_autotune", "pcsf_set_variant", "pcsf_score",
     "pcsf_model_specialized", "pcsf_batch_create_frames", "pcsf_simulate_nucleotides",
-    "pcsf_program", "pcsf_batch_codes",
+    "pcsf_program", "pcsf_batch_codes", "pcsf_forward_noalloc",
 ]
 PCSF_STRATEGY_FIXED = 0
 PCSF_STRATEGY_MLE = 1
@@ -108,6 +109,7 @@ def lib():
     L.pcsf_batch_codes.argtypes = [vp, u8p, i64p]
     L.pcsf_forward.argtypes = [vp, dp, C.c_int, C.c_int, dp, dp, C.c_int]
     L.pcsf_backward.argtypes = [vp, dp, dp, dp]
+    L.pcsf_forward_noalloc.argtypes = [vp, vp, C.c_int64, vp, C.c_int32, dp, C.c_int, vp, vp]
     L.pcsf_batch_device_results.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
     L.pcsf_batch_last_timing.argtypes = [vp, dp, dp, dp]
     L.pcsf_lik_columns.argtypes = [vp, C.c_int, C.c_int, i32p, dp, dp, i32p, C.c_int64, dp, dp]

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 
 #include "host/maximize.hpp"
@@ -170,22 +171,25 @@ int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int 
   return PCSF_OK;
 }
 
-// Transition sets and cherry tables of the fixed scales `rho` (one per
-// model) with P tiles in format fmt; rebuilt only when either changes.
-int ensure_transitions(pcsf_model* m, const std::vector<double>& rho, int fmt, float* ms_out) {
+// Transition sets and cherry tables of the fixed scales rho[0 .. nm) (one
+// per model) with P tiles in format fmt; rebuilt only when either changes.
+// Allocates nothing: the sets are the model's own and cached_rho's capacity
+// is reserved when the model is made.
+int ensure_transitions(pcsf_model* m, const double* rho, int fmt, float* ms_out) {
   pcsf_ctx* c = m->ctx;
   if (ms_out) *ms_out = 0.f;
-  if (m->cached_rho == rho && m->fixed.fmt == fmt) return PCSF_OK;
-  const bool same_rho = m->cached_rho == rho;
+  const bool same_rho = (int)m->cached_rho.size() == m->nm &&
+                        std::equal(rho, rho + m->nm, m->cached_rho.begin());
+  if (same_rho && m->fixed.fmt == fmt) return PCSF_OK;
   m->cached_rho.clear();
-  HIP_TRY(c, hipMemcpyAsync(m->fixed.d_rho, rho.data(), sizeof(double) * rho.size(),
-                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(m->fixed.d_rho, rho, sizeof(double) * m->nm, hipMemcpyHostToDevice,
+                            c->stream));
   HIP_TRY(c, hipMemsetAsync(m->fixed.d_active, 1, (size_t)m->nm, c->stream));
-  int rc = run_pbuild(m, m->fixed, m->nm, rho.data(), fmt, ms_out);
+  int rc = run_pbuild(m, m->fixed, m->nm, rho, fmt, ms_out);
   if (rc) return rc;
   if (!same_rho) HIP_TRY(c, launch_cherry_tables(m, m->fixed, m->nm, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  m->cached_rho = rho;
+  m->cached_rho.assign(rho, rho + m->nm);
   return PCSF_OK;
 }
 
@@ -380,6 +384,55 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
   return PCSF_OK;
 }
 
+// ---- scratch of pcsf_forward_noalloc ---------------------------------------
+// One fixed device array per GPU, held until the process ends: the
+// leaf-major codes, the per-column ln z and elpr of a call, and a status
+// word.  On the host the name `scratch` stands for a host-side shadow, not a
+// device address, so host code never hands it to a kernel or a copy; each
+// device's address comes from hipGetSymbolAddress, taken once per device
+// with that device current (pcsf_ctx_create) and kept in g_scratch.
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[PCSF_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};          // device address of `scratch`, per device
+hipError_t g_scratch_err[kMaxDevices] = {};  // status of that lookup
+bool g_scratch_done[kMaxDevices] = {};
+
+// Looks the array up on `device` (which must be current) the first time;
+// afterwards returns the kept address.
+void* scratch_address(int device, hipError_t* err) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  if (!g_scratch_done[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch[device] = (e == hipSuccess) ? ptr : nullptr;
+    g_scratch_err[device] = e;
+    g_scratch_done[device] = true;
+  }
+  *err = g_scratch_err[device];
+  return g_scratch[device];
+}
+
+// Region offsets on the device: off[0] = 0, nondecreasing, off[nreg] =
+// ncols; a violation sets bit 1 of *status.
+__global__ void noalloc_check_offsets_kernel(const int64_t* __restrict__ off, int nreg,
+                                             int64_t ncols, int* __restrict__ status) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > nreg) return;
+  bool bad = (r == 0) ? off[0] != 0 : off[r] < off[r - 1];
+  if (r == nreg) bad = bad || off[nreg] != ncols;
+  if (bad) atomicOr(status, 2);
+}
+
 }  // namespace
 
 extern "C" {
@@ -400,6 +453,10 @@ int pcsf_ctx_create(int device, pcsf_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     c->num_cus = prop.multiProcessorCount;
+  // the no-allocation forward's scratch on this device (looked up once per
+  // device; a failure is reported by pcsf_forward_noalloc)
+  hipError_t se = hipSuccess;
+  (void)scratch_address(device, &se);
   *out = c;
   return PCSF_OK;
 }
@@ -555,6 +612,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
         }
       }
   m->eig.resize(nm);
+  m->cached_rho.reserve(nm);
   m->pi.assign(pi, pi + 64 * nm);
   std::vector<double> V(nm * 4096), Vi(nm * 4096), lam(nm * 64), lnpi(nm * 64), QT(nm * 4096);
   for (int q = 0; q < nm; ++q) {
@@ -689,7 +747,7 @@ int pcsf_model_transition(pcsf_model* m, int model, double rho, double* P_out) {
   if (!rho_valid(rho))
     return set_err(c, PCSF_EINVAL, "CamlPaml.P14n.instantiate_tree: domain violation on variable 0");
   std::vector<double> r(m->nm, rho);
-  int rc = ensure_transitions(m, r, m->fixed.fmt < 0 ? kFragF32 : m->fixed.fmt, nullptr);
+  int rc = ensure_transitions(m, r.data(), m->fixed.fmt < 0 ? kFragF32 : m->fixed.fmt, nullptr);
   if (rc) return rc;
   std::vector<double> T((size_t)m->nbr * 4096);
   HIP_TRY(c, hipMemcpy(T.data(), m->fixed.d_P64T + (size_t)model * m->nbr * 4096, T.size() * 8,
@@ -898,7 +956,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   float tp = 0.f;
   const int fmt = precision == PCSF_FP64 ? (m->fixed.fmt < 0 ? kFragF32 : m->fixed.fmt)
                                          : frag_format(precision);
-  int rc = ensure_transitions(m, r, fmt, &tp);
+  int rc = ensure_transitions(m, r.data(), fmt, &tp);
   if (rc) return rc;
   b->t_pbuild = tp;
   b->last_rho.resize((size_t)b->nreg * m->nm);
@@ -933,6 +991,100 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   return PCSF_OK;
 }
 
+int pcsf_forward_noalloc(pcsf_model* m, const uint8_t* d_codes, int64_t n_cols,
+                         const int64_t* d_region_offsets, int32_t n_regions, const double* rho,
+                         int precision, double* d_loglik, double* d_elpr) {
+  if (!m || !m->ctx || !rho) return PCSF_EINVAL;
+  pcsf_ctx* c = m->ctx;
+  if (precision < PCSF_FP64 || precision > PCSF_BF16)
+    return set_err(c, PCSF_EINVAL, "pcsf_forward_noalloc: unknown precision");
+  if (n_cols < 0 || n_regions < 0 || (n_cols > 0 && !d_codes) || !d_region_offsets ||
+      (n_regions > 0 && (!d_loglik || !d_elpr)))
+    return set_err(c, PCSF_EINVAL, "pcsf_forward_noalloc: bad arguments");
+  for (int q = 0; q < m->nm; ++q)
+    if (!rho_valid(rho[q]))
+      return set_err(c, PCSF_EINVAL,
+                     "CamlPaml.P14n.instantiate_tree: domain violation on variable 0");
+  // scratch layout: leaf-major codes (nl x ld bytes), ln z and elpr per
+  // column (nm x ld doubles each), status word
+  constexpr size_t kCap = PCSF_NOALLOC_SCRATCH_BYTES;
+  const size_t per_col = (size_t)m->nl + 16 * (size_t)m->nm;
+  if ((size_t)n_cols > kCap / per_col) {
+    char buf[200];
+    std::snprintf(buf, sizeof buf,
+                  "pcsf_forward_noalloc: %lld columns of a %d-leaf, %d-model tree exceed the "
+                  "%zu-byte scratch (at most %zu columns)",
+                  (long long)n_cols, m->nl, m->nm, kCap, kCap / per_col - kGroupCols);
+    return set_err(c, PCSF_EINVAL, buf);
+  }
+  const int64_t ld = std::max<int64_t>(kGroupCols, (n_cols + kGroupCols - 1) / kGroupCols *
+                                                       kGroupCols);
+  const size_t codes_bytes = ((size_t)m->nl * ld + 255) / 256 * 256;
+  const size_t col_bytes = (size_t)m->nm * ld * sizeof(double);
+  if (codes_bytes + 2 * col_bytes + 256 > kCap) {
+    char buf[200];
+    std::snprintf(buf, sizeof buf,
+                  "pcsf_forward_noalloc: %lld columns of a %d-leaf, %d-model tree exceed the "
+                  "%zu-byte scratch (at most %zu columns)",
+                  (long long)n_cols, m->nl, m->nm, kCap, kCap / per_col - kGroupCols);
+    return set_err(c, PCSF_EINVAL, buf);
+  }
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipError_t se = hipSuccess;
+  unsigned char* base = static_cast<unsigned char*>(scratch_address(c->device, &se));
+  if (!base)
+    return set_err(c, PCSF_EDEVICE,
+                   std::string("pcsf_forward_noalloc: scratch lookup failed: ") +
+                       hipGetErrorString(se));
+  // transition sets at these scales: the model's own (allocated when it was made)
+  const int fmt = precision == PCSF_FP64 ? (m->fixed.fmt < 0 ? kFragF32 : m->fixed.fmt)
+                                         : frag_format(precision);
+  int rc = ensure_transitions(m, rho, fmt, nullptr);
+  if (rc) return rc;
+  // a view of the scratch as a batch (owns nothing; its vectors stay empty)
+  pcsf_batch v;
+  v.m = m;
+  v.ncols = n_cols;
+  v.ld = ld;
+  v.nreg = n_regions;
+  v.d_codes = base;
+  v.d_offsets = const_cast<int64_t*>(d_region_offsets);
+  v.d_logz = reinterpret_cast<double*>(base + codes_bytes);
+  v.d_colelpr = reinterpret_cast<double*>(base + codes_bytes + col_bytes);
+  v.d_loglik = d_loglik;
+  v.d_elpr = d_elpr;
+  int* d_status = reinterpret_cast<int*>(base + codes_bytes + 2 * col_bytes);
+  // leaf codes into the scratch, checked; region offsets checked
+  HIP_TRY(c, hipMemsetAsync(d_status, 0, sizeof(int), c->stream));
+  HIP_TRY(c, launch_transpose_codes(d_codes, n_cols, m->nl, v.d_codes, ld, d_status, c->stream));
+  hipLaunchKernelGGL(noalloc_check_offsets_kernel, dim3((unsigned)((n_regions + 256) / 256)),
+                     dim3(256), 0, c->stream, d_region_offsets, (int)n_regions, n_cols,
+                     d_status);
+  HIP_TRY(c, hipGetLastError());
+  int status = 0;
+  HIP_TRY(c, hipMemcpyAsync(&status, d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (status & 2)
+    return set_err(c, PCSF_EINVAL,
+                   "pcsf_forward_noalloc: region offsets must span [0, n_cols] and not decrease");
+  if (status & 1)
+    return set_err(c, PCSF_EINVAL,
+                   "pcsf_forward_noalloc: a leaf code is neither a codon index (0..63) nor "
+                   "PCSF_MARGINALIZE (255)");
+  hipError_t e;
+  if (precision == PCSF_FP64) {
+    e = launch_prune_fp64(m, &v, c->stream);
+    if (e == hipErrorInvalidValue)
+      return set_err(c, PCSF_EINVAL, "FP64 kernel: tree workspace exceeds 64 KiB of LDS");
+  } else {
+    e = launch_prune_mfma(m, &v, precision, m->mfma_variant[precision], c->stream);
+  }
+  HIP_TRY(c, e);
+  HIP_TRY(c, launch_region_reduce(&v, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PCSF_OK;
+}
+
 int pcsf_autotune(pcsf_batch* b, int precision, int reps, double* ms_out, int* chosen) {
   if (!b || reps < 1) return PCSF_EINVAL;
   pcsf_model* m = b->m;
@@ -942,7 +1094,7 @@ int pcsf_autotune(pcsf_batch* b, int precision, int reps, double* ms_out, int* c
   HIP_TRY(c, hipSetDevice(c->device));
   {
     std::vector<double> one(m->nm, 1.0);
-    int rc = ensure_transitions(m, one, frag_format(precision), nullptr);
+    int rc = ensure_transitions(m, one.data(), frag_format(precision), nullptr);
     if (rc) return rc;
   }
   hipEvent_t e0, e1;
