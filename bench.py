@@ -135,16 +135,21 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="threads of the CPU baseline (the GPU host's share is 16 cores)")
     ap.add_argument("--regions", type=int, default=None, help="override region count")
-    ap.add_argument("--precision", choices=sorted(PRECISIONS), default=None,
+    ap.add_argument("--precision", choices=sorted(PRECISIONS) + ["fp64"], default=None,
                     help="GEMM precision of the batched MFMA path (fp16s and fp32 meet "
                          "the 1e-3 dB bound; fp16 and bf16 do not); default fp16s, and "
                          "fp32 for C4 (its configuration names FP32)")
-    ap.add_argument("--workload", choices=["C2", "C3", "C4", "C5"], default="C2",
+    ap.add_argument("--workload", choices=["C2", "C3", "C4", "C5", "omega", "estep"],
+                    default="C2",
                     help="C2 (default) is the headline measurement; C3/C4/C5 print their own "
-                         "line for the other scoring configurations")
+                         "line for the other scoring configurations, omega the omega test's "
+                         "and estep PhyloEM's expected substitution counts")
     args = ap.parse_args()
     if args.precision is None:
         args.precision = "fp32" if args.workload == "C4" else "fp16s"
+    if args.precision == "fp64" and args.workload != "omega":
+        ap.error("--precision fp64 is for --workload omega (the other lines time the MFMA paths "
+                 "and report the plain FP64 kernel beside them)")
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (regions shard "
               "across GPUs by design only); run with --gpus 1", file=sys.stderr)
@@ -191,6 +196,28 @@ def main():
     log("%s MFMA: %.3f ms/step, %.3e col/s, kernel %.3f ms" % (args.precision, ms_per_step, value,
                                                               np.mean(kernel_ms)))
 
+    # ---- the same batch at FP32 (the configuration's stated GEMM precision)
+    fp32 = None
+    if args.precision != "fp32":
+        batch.forward(1.0, P.PCSF_FP32)
+        c32, _ = batch.autotune(reps=2, precision=P.PCSF_FP32)
+        for _ in range(args.warmup):
+            batch.forward(1.0, P.PCSF_FP32)
+        ctx.synchronize()
+        k32 = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            llf, _ = batch.forward(1.0, P.PCSF_FP32)
+            k32.append(batch.timing()["prune_ms"])
+        ctx.synchronize()
+        el32 = time.perf_counter() - t0
+        fp32 = {"value": ncols * args.steps / el32, "unit": "codon-columns/s",
+                "ms_per_step": 1000 * el32 / args.steps, "kernel_ms": float(np.mean(k32)),
+                "launch_shape": int(c32),
+                "achieved_tflops": 16384.0 * (nl - 2) * ncols / (np.mean(k32) / 1000) / 1e12,
+                "peak_tflops": PEAK_FP32_TFLOPS}
+        log("fp32 MFMA: %.3f ms/step, %.3e col/s" % (fp32["ms_per_step"], fp32["value"]))
+
     # ---- plain FP64 GPU kernel on the same batch (the 50x denominator)
     fp64_rate, dmax = None, None
     if args.fp64_steps > 0:
@@ -205,6 +232,9 @@ def main():
         s32 = P.DB * (ll32[:, 0] - ll32[:, 1])
         s64 = P.DB * (ll64[:, 0] - ll64[:, 1])
         dmax = float(np.max(np.abs(s32 - s64))) if len(s32) else 0.0
+        if fp32 is not None:
+            fp32["max_abs_delta_score_db_vs_fp64"] = float(
+                np.max(np.abs(P.DB * (llf[:, 0] - llf[:, 1]) - s64)))
 
     # ---- roofline of the dominant kernel (fused pruning: MFMA-bound; its HBM
     # traffic is the codes plus per-column results, ~nl + 32 B per column)
@@ -266,6 +296,7 @@ def main():
         "plain_fp64_gpu": {"value": fp64_rate, "unit": "codon-columns/s",
                            "speedup_of_value": value / fp64_rate if fp64_rate else None},
         "accuracy": {"max_abs_delta_score_db_vs_fp64": dmax, "bound_db": 1e-3},
+        "fp32": fp32,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -297,6 +328,8 @@ def other_workload(args):
     ctx = P.Context(0)
     ctx.set_timing(True)
     name = args.workload
+    if name in ("omega", "estep"):
+        return extra_workload(args, P, W, ctx)
     model = W.model_for(name, ctx)
     prec, dtype, _, _ = PRECISIONS[args.precision]
     steps = max(1, args.steps)
@@ -354,6 +387,9 @@ def other_workload(args):
             log("%s: %.2f s, %.3e scored col/s, %d evaluations per region (mean)"
                 % (strat, dt, wl.n_cols / dt, r["n_evals"].mean()))
         r64 = P.score(batch, "mle", P.PCSF_FP64)
+        t1 = time.perf_counter()
+        g64 = P.score(batch, "mle_grad", P.PCSF_FP64)
+        dt64 = time.perf_counter() - t1
         g, dtg = res["mle_grad"]
         b, dtb = res["mle"]
         # finite-difference check of d lpr / d rho at the gradient estimates
@@ -381,8 +417,12 @@ def other_workload(args):
                                    "evaluations_per_s": float(wl.n_cols * b["n_evals"].mean() / dtb),
                                    "mean_evaluations": float(b["n_evals"].mean())},
               mean_gradient_evaluations=float(g["n_evals"].mean()),
-              max_abs_delta_score_db={"mle_grad_vs_fp64_mle": float(np.max(np.abs(g["score"] - r64["score"]))),
-                                      "mle_vs_fp64_mle": float(np.max(np.abs(b["score"] - r64["score"])))},
+              max_abs_delta_score_db={
+                  "mle_grad_vs_fp64_mle_grad": float(np.max(np.abs(g["score"] - g64["score"]))),
+                  "mle_grad_vs_fp64_mle": float(np.max(np.abs(g["score"] - r64["score"]))),
+                  "mle_vs_fp64_mle": float(np.max(np.abs(b["score"] - r64["score"])))},
+              fp64_gradient_maximiser={"scored_columns_per_s": wl.n_cols / dt64,
+                                       "mean_gradient_evaluations": float(g64["n_evals"].mean())},
               lpr_gain_of_gradient_maximiser=float(np.max(g["lpr"] - r64["lpr"])),
               fd_rel_error_of_d_lpr_d_rho=fd_err,
               gradient_rel_error_vs_fp64_adjoint=grad_err)
@@ -433,6 +473,100 @@ def other_workload(args):
                            "mean_gradient_evaluations": float(g["n_evals"].mean()),
                            "per_branch_gradient_columns_per_s": sub_cols / dt_b,
                            "branches": int(gbr.shape[-1])})
+
+
+def extra_workload(args, P, W, ctx):
+    """The rows next to the scoring path, one JSON line each:
+      omega  the omega test (OmegaModel.score) on 58mammals regions simulated as
+             the C2 workload, lengths U{50..500}: every region's two MAP fits in
+             lock step on the GPU; scored codon-columns/s, evaluations per region,
+             error against the FP64 run, and the FP64 oracle on a sample;
+      estep  PhyloEM's expected substitution counts (collect_sufficient_statistics,
+             every branch posterior summed over columns) for the 58mammals coding
+             model over C2 columns: codon-columns/s against the FP64 oracle's.
+    """
+    from oracle import oracle as O
+    from phylocsf_amd.api import paramset_prefix
+    prefix = paramset_prefix("58mammals")
+    tree = P.newick_tree(open(prefix + ".nh").read())
+    otree = O.newick_tree(open(prefix + ".nh").read())
+    sim = P.PhyloCSFModel.from_paramset(None, "58mammals")
+    if args.workload == "omega":
+        R = args.regions or 200
+        rng = np.random.default_rng(6)
+        lengths = rng.integers(50, 501, R)
+        codes, off = sim.simulate(lengths, rng.integers(0, 2, R), np.ones(R), gap_p=0.1,
+                                  n_p=0.01, seed=6, n_threads=16)
+        batch = P.Batch(P.PhyloCSFModel.tree_only(ctx, tree), codes, off)
+        prec, dtype, _, _ = PRECISIONS[args.precision] if args.precision != "fp64" else \
+            (0, "fp64", None, None)
+        t0 = time.perf_counter()
+        r = P.score_omega(batch, precision=prec)
+        dt = time.perf_counter() - t0
+        log("omega (%s): %.2f s, %.3e scored col/s" % (args.precision, dt, off[-1] / dt))
+        err = None
+        if prec != 0:
+            r64 = P.score_omega(batch, precision=0)
+            err = float(np.max(np.abs(r["score"] - r64["score"])))
+        # FP64 oracle on the first regions, about cpu-seconds of one thread
+        t0 = time.perf_counter()
+        nr = nc = 0
+        while nr < R and time.perf_counter() - t0 < args.cpu_seconds:
+            O.omega_score(otree, codes[off[nr]:off[nr + 1]])
+            nc += int(off[nr + 1] - off[nr])
+            nr += 1
+        cdt = time.perf_counter() - t0
+        _line("scored codon-columns/s, omega test (two MAP fits per region)", off[-1] / dt,
+              "codon-columns/s", 1, 0, 1000 * dt, dtype,
+              {"workload": "58mammals tree, %d regions U{50..500} simulated under the C2 "
+                           "models (gaps 0.1, 1%% N), omega test, H1 omega 0.2 sigma 0.01 "
+                           "(seed 6)" % R, "codon_columns": int(off[-1])},
+              mean_evaluations_per_region=float(r["n_evals"].mean()),
+              failed_regions=int((r["status"] != 0).sum()),
+              max_abs_delta_score_db_vs_fp64=err,
+              cpu_baseline={"value": nc / cdt, "unit": "codon-columns/s", "cores": 1,
+                            "kind": "port",
+                            "sample": "regions 0..%d (%d codon columns), FP64 oracle, %.1f s "
+                                      "on 1 thread of %s" % (nr - 1, nc, cdt, cpu_model())})
+        return
+    # estep
+    wl = W.make("C2", sim, n_regions=args.regions or 60, n_threads=16)
+    leaves = np.where(wl.codes == 255, -1, wl.codes.astype(np.int32))
+    S, pi = O.ecm_read(prefix + "_coding.ECM")
+    Q = O.build_q(S, pi)
+    Pm = np.stack([O.pt(Q, pi, t) for t in otree.branch[:-1]])
+    n = tree.n
+    lik = P.PhyloLik(ctx)
+    it = lik.prepare(tree, Pm, pi, leaves[:100])
+    it.add_branch_posteriors(np.zeros((n - 1, 64, 64)))  # warm-up
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ss = np.zeros((n - 1, 64, 64))
+        P.collect_sufficient_statistics(ctx, tree, Pm, pi, leaves, ss)
+    dt = (time.perf_counter() - t0) / steps
+    ncols = leaves.shape[0]
+    log("E-step: %d columns in %.3f s, %.3e col/s" % (ncols, dt, ncols / dt))
+    t0 = time.perf_counter()
+    nsmp = 0
+    while nsmp < ncols and time.perf_counter() - t0 < args.cpu_seconds:
+        O.lik_posteriors(otree, Pm, pi, leaves[nsmp:nsmp + 50], posteriors=False)
+        nsmp += 50
+    cdt = time.perf_counter() - t0
+    _z, _p, ec0 = O.lik_posteriors(otree, Pm, pi, leaves[:200], posteriors=False)
+    ss = np.zeros((n - 1, 64, 64))
+    P.collect_sufficient_statistics(ctx, tree, Pm, pi, leaves[:200], ss)
+    err = float(np.max(np.abs(ss - ec0)) / np.max(np.abs(ec0)))
+    _line("codon-columns/s, PhyloEM expected substitution counts (every branch)", ncols / dt,
+          "codon-columns/s", steps, 1, 1000 * dt, "fp64",
+          {"workload": "58mammals coding model, %d columns of the C2 workload (seed 2), "
+                       "branch posteriors of all %d branches summed over columns"
+                       % (ncols, n - 1), "codon_columns": ncols},
+          max_error_vs_oracle_first_200_columns_relative_to_max_count=err,
+          cpu_baseline={"value": nsmp / cdt, "unit": "codon-columns/s", "cores": 1,
+                        "kind": "port",
+                        "sample": "first %d columns, FP64 oracle, %.1f s on 1 thread of %s"
+                                  % (nsmp, cdt, cpu_model())})
 
 
 if __name__ == "__main__":

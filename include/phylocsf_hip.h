@@ -100,6 +100,8 @@ int pcsf_program(int n_nodes, const int32_t *children, int with_tables, int32_t 
  * S is n_models*64*64, pi is n_models*64, branch has n_nodes entries (the
  * root's is ignored).  Every S must be symmetric (a reversible model, as
  * ECM.import_parameters builds it); PCSF_EINVAL otherwise.
+ * n_models = 0 (S, pi may be NULL) makes a tree-only model for
+ * pcsf_score_omega, whose sets carry rate models of their own.
  * ctx may be NULL: a host-only model, for pcsf_simulate,
  * pcsf_simulate_nucleotides and pcsf_model_info (no device memory; the
  * device entry points refuse it with PCSF_EINVAL). */
@@ -204,6 +206,10 @@ int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
  *   PCSF_STRATEGY_MLE    llr_MaxLik ~init:1 (src/PhyloCSFModel.ml:84-99,
  *                        130-136): Fit.find_init then GSL Brent per region and
  *                        model, all regions advancing in lock step on the GPU
+ * At an MFMA precision (PCSF_FP32 .. PCSF_BF16) the maximisers search on
+ * that precision's likelihoods and take lpr and elpr at their estimates from
+ * one FP64 forward (the MFMA rounding, up to ~6e-9 of lpr on regions of
+ * thousands of codons at per-region scales, would otherwise reach the score).
  * score / anc_comp in decibans (n_regions); rho, lpr, elpr per region and
  * model (n_regions*2, coding first); status per region (PCSF_OK, or
  * PCSF_EFAIL where the reference raises inside the maximiser);
@@ -220,6 +226,27 @@ int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
 #define PCSF_STRATEGY_MLE_GRAD 2
 int pcsf_score(pcsf_batch *b, int strategy, int precision, double *score, double *anc_comp,
                double *rho, double *lpr, double *elpr, int32_t *status, int32_t *n_evals);
+
+/* The omega (dN/dS) test, OmegaModel.score (src/OmegaModel.ml:195-219) for
+ * every region of a batch: H0 (omega = sigma = 1) and H1 (omega = omega_h1,
+ * sigma = sigma_h1; the reference's defaults are 0.2 and 0.01, a parameter
+ * set's `_omega` file may name others, src/PhyloCSF.ml:452-463), each the
+ * MAP fit kr_map (:160-190): three rounds of maximize_lpr over the tree scale
+ * in [0.001, 10] (half-Cauchy prior, mode 1, scale 0.5) then over kappa in
+ * [1, 10] (gamma(7, 0.25) prior on kappa - 1), with F3x4 codon frequencies
+ * counted from the region (:102-134).  The codon model (:21-80) is rebuilt
+ * and diagonalised per (region, hypothesis, kappa) on the host; every
+ * likelihood is a batched per-region forward on the GPU.
+ * The batch's model is a tree-only model (pcsf_model_create with n_models =
+ * 0) or any two-model model (its rate models are not used).
+ * score in decibans (n_regions; anc_comp is not defined for this test);
+ * lpr (log prior + lpr at the estimates), rho, kappa per region and
+ * hypothesis (n_regions * 2, H0 first); status per region (PCSF_EFAIL where
+ * the reference raises); n_evals = likelihood evaluations per region.
+ * Any output may be NULL. */
+int pcsf_score_omega(pcsf_batch *b, double omega_h1, double sigma_h1, int precision,
+                     double *score, double *lpr, double *rho, double *kappa, int32_t *status,
+                     int32_t *n_evals);
 
 /* Autotuner of the fused kernel at one MFMA precision (PCSF_FP32 ..
  * PCSF_BF16): times every launch shape (waves per workgroup sharing one LDS
@@ -257,13 +284,32 @@ int pcsf_batch_last_timing(const pcsf_batch *b, double *prune_ms, double *reduce
 
 /* PhyloLik.prepare + likelihood + node_posterior (lib/CamlPaml/PhyloLik.ml:
  * 46-138) over n_cols columns with caller-supplied P matrices, any alphabet
- * size k <= 64:
+ * size k <= 64 and any tree size:
  *   P[(n_nodes-1)*k*k] row-major, P[br] for the branch leading to node br;
  *   prior[k]; leaves[n_cols*nl] state index or -1 (`Marginalize).
  * Outputs z[n_cols] and, if non-NULL, root_posterior[n_cols*k].  FP64. */
 int pcsf_lik_columns(pcsf_ctx *ctx, int k, int n_nodes, const int32_t *children,
                      const double *P, const double *prior, const int32_t *leaves,
                      int64_t n_cols, double *z, double *root_posterior);
+
+/* The rest of PhyloLik (lib/CamlPaml/PhyloLik.mli:6-35) over n_cols columns,
+ * FP64, any k <= 64 and tree size:
+ *   leaves[n_cols*nl]: state index (`Certain; >= k gives a leaf vector of
+ *     zeros, as the reference's), -1 (`Marginalize) or -2 (`Distribution,
+ *     its k probabilities at leaf_dist[(col*nl + leaf)*k ..]);
+ *   z[n_cols]: likelihood;
+ *   node_post (may be NULL) [n_cols][n_nodes][k]: node_posterior of every
+ *     node (a leaf's is its leaf vector; all zero where z = 0);
+ *   ecounts (may be NULL) [(n_nodes-1)][k][k]: add_branch_posteriors of every
+ *     branch (:140-174) with weight weights[col] (NULL: 1), summed over the
+ *     columns and ADDED to what the array holds -- the E-step of
+ *     PhyloEM.collect_sufficient_statistics (lib/CamlPaml/PhyloEM.ml:14-20):
+ *     ecounts[br][a][b] += w P(parent = a, br = b | leaves), computed as one
+ *     rank-n_cols update per branch. */
+int pcsf_lik_posteriors(pcsf_ctx *ctx, int k, int n_nodes, const int32_t *children,
+                        const double *P, const double *prior, const int32_t *leaves,
+                        const double *leaf_dist, int64_t n_cols, const double *weights,
+                        double *z, double *node_post, double *ecounts);
 
 /* ---- synthetic alignments (bench / tests) ----------------------------- */
 

@@ -15,6 +15,8 @@ Restates src/PhyloCSF.ml (paths relative to the mlin/PhyloCSF tree):
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from . import oracle as O
@@ -196,14 +198,37 @@ def translate(dna):
     return "".join(out)
 
 
+def omega_config(param_prefix):
+    """initialize_strategy's OmegaTest (src/PhyloCSF.ml:452-463): (omega_H1,
+    sigma_H1) from the optional key<TAB>value file <prefix>_omega."""
+    fn = param_prefix + "_omega"
+    if not os.path.exists(fn):
+        return 0.2, 0.01
+    kv = {}
+    for line in open(fn).read().split("\n"):
+        line = line.strip()
+        if not line or line[0] == "#":
+            continue
+        k, _, v = line.partition("\t")
+        kv[k] = v
+    try:
+        return float(kv["omega_H1"]), float(kv["sigma_H1"])
+    except (KeyError, ValueError):
+        raise CliError("configuration file %s exists but does not specify valid omega_H1 and "
+                       "sigma_H1 values" % fn)
+
+
 def run(param_prefix, mfa_text, strategy="mle", frames=1, orf="AsIs", min_codons=25,
         remove_ref_gaps_=False, species=None):
     """process_alignment for one MFA; returns list of (score dict, rc, lo, hi) and the max."""
     nh = open(param_prefix + ".nh").read()
     tree = O.newick_tree(nh, keep=species)
-    Sc, pic = O.ecm_read(param_prefix + "_coding.ECM")
-    Sn, pin = O.ecm_read(param_prefix + "_noncoding.ECM")
-    model = O.Model(tree, [Sc, Sn], [pic, pin])
+    if strategy == "omega":
+        h1 = omega_config(param_prefix)
+    else:
+        Sc, pic = O.ecm_read(param_prefix + "_coding.ECM")
+        Sn, pin = O.ecm_read(param_prefix + "_noncoding.ECM")
+        model = O.Model(tree, [Sc, Sn], [pic, pin])
     sp, seqs = input_mfa(mfa_text)
     if remove_ref_gaps_:
         seqs = remove_ref_gaps(seqs)
@@ -227,6 +252,9 @@ def run(param_prefix, mfa_text, strategy="mle", frames=1, orf="AsIs", min_codons
         codes = pleaves(rc_seqs if rc else seqs, leaf_ord, lo, hi)
         if strategy == "mle":
             s = model.score_mle(codes)
+        elif strategy == "omega":
+            s = O.omega_score(tree, codes, *h1)
+            s["anc_comp"] = float("nan")
         else:
             s = model.score_fixed(codes)
         results.append((s, rc, lo, hi))
@@ -241,4 +269,6 @@ def run(param_prefix, mfa_text, strategy="mle", frames=1, orf="AsIs", min_codons
 
 def _key(r):
     s, rc, lo, hi = r
-    return (s["score"], s["anc_comp"], rc, lo, hi)
+    # OCaml's compare takes nan as equal to nan (the omega test's anc_comp_score)
+    a = s["anc_comp"]
+    return (s["score"], 0.0 if a != a else a, rc, lo, hi)

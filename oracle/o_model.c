@@ -343,6 +343,66 @@ int o_prune(int k, int n, const int *lc, const int *rc, const double *P, const d
   return 0;
 }
 
+/* PhyloLik.prepare with any leaf kind, likelihood, node_posterior of every
+ * node and add_branch_posteriors of every branch summed over the columns with
+ * weights -- PhyloEM.collect_sufficient_statistics (lib/CamlPaml/PhyloEM.ml:
+ * 14-20).  leaves[c*nl + j]: state index, -1 `Marginalize, -2 `Distribution
+ * (leaf_dist[(c*nl + j)*k + x]).  weights (NULL: 1).  z[ncols]; post
+ * (optional) [ncols][n][k]; ecounts (optional) [(n-1)][k][k], added to. */
+int o_lik_posteriors(int k, int n, const int *lc, const int *rc, const double *P,
+                     const double *prior, const int *leaves, const double *leaf_dist,
+                     int64_t ncols, const double *weights, double *z, double *post,
+                     double *ecounts) {
+  o_ws W;
+  ws_init(&W, k, n, lc, rc);
+  const int nl = W.nl;
+  for (int64_t c = 0; c < ncols; c++) {
+    /* get_leaf (PhyloLik.ml:16-19) */
+    for (int j = 0; j < nl; j++) {
+      double *a = W.alpha + (size_t)j * k;
+      const int v = leaves[c * nl + j];
+      for (int x = 0; x < k; x++)
+        a[x] = (v == -2) ? leaf_dist[((size_t)c * nl + j) * k + x]
+               : (v < 0) ? 1. : (x == v ? 1. : 0.);
+    }
+    const double zc = inside(&W, P, prior);
+    z[c] = zc;
+    if (!post && !ecounts) continue;
+    outside(&W, P, prior);
+    if (post) {
+      double *pc = post + (size_t)c * n * k;
+      for (int i = 0; i < n; i++)
+        for (int x = 0; x < k; x++)
+          pc[(size_t)i * k + x] = (zc == 0.) ? 0.
+                                  : (i < nl) ? W.alpha[(size_t)i * k + x]
+                                             : W.alpha[(size_t)i * k + x] *
+                                                   W.beta[(size_t)i * k + x] / zc;
+    }
+    if (ecounts && zc > 0.) {
+      /* add_branch_posteriors ?weight (PhyloLik.ml:140-174) for every branch */
+      const double w = weights ? weights[c] : 1.0;
+      for (int br = 0; br < n - 1; br++) {
+        const int p = W.parent[br], sib = W.sib[br];
+        const double *sm = P + (size_t)br * k * k, *sms = P + (size_t)sib * k * k;
+        const double *bp = W.beta + (size_t)p * k, *ab = W.alpha + (size_t)br * k;
+        const double *xas = W.alpha + (size_t)sib * k;
+        double *ec = ecounts + (size_t)br * k * k;
+        for (int a = 0; a < k; a++) {
+          const double bpa = bp[a];
+          if (!(bpa > 0.)) continue;
+          const double bpa_sibtot = bpa * ddot(k, sms + (size_t)a * k, 1, xas, 1);
+          for (int b = 0; b < k; b++) {
+            const double pr = bpa_sibtot * sm[(size_t)a * k + b] * ab[b] / zc;
+            ec[(size_t)a * k + b] = ec[(size_t)a * k + b] + w * pr;
+          }
+        }
+      }
+    }
+  }
+  ws_free(&W);
+  return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* PhyloCSFModel (src/PhyloCSFModel.ml)                                     */
 
@@ -561,4 +621,302 @@ int o_score_mle(o_model *m, const uint8_t *codes, int64_t ncols, double *out, ch
   out[5] = rn;
   out[6] = (double)evals;
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* OmegaModel (src/OmegaModel.ml): the omega (dN/dS) test                   */
+
+/* Code.Codon64.translation_table (lib/CamlPaml/Code.ml:153-173), codon index
+ * 16 n1 + 4 n2 + n3 with A, C, G, T = 0..3; stops TAA, TAG, TGA. */
+static const char kAA[65] = "KNKNTTTTRSRSIIMIQHQHPPPPRRRRLLLLEDEDAAAAGGGGVVVV*Y*YSSSS*CWCLFLF";
+static int is_stop_index(int c) { return c == 48 || c == 50 || c == 56; }
+
+/* pi_exprs (OmegaModel.ml:24-40): settings qs[0..11] = kappa, omega, sigma,
+ * then per codon position the frequencies of A, C, G relative to T.  The
+ * values are not normalised (the stop-codon term only divides them all). */
+void o_omega_pi(const double *qs, double *pi) {
+  double sc[64];
+  const double d1 = qs[3] + (qs[4] + (qs[5] + 1.0));
+  const double d2 = qs[6] + (qs[7] + (qs[8] + 1.0));
+  const double d3 = qs[9] + (qs[10] + (qs[11] + 1.0));
+  for (int c = 0; c < 64; c++) {
+    const int i1 = c / 16, i2 = (c / 4) % 4, i3 = c % 4;
+    const double f1 = (i1 == 3 ? 1.0 : qs[3 + i1]) / d1;
+    const double f2 = (i2 == 3 ? 1.0 : qs[6 + i2]) / d2;
+    const double f3 = (i3 == 3 ? 1.0 : qs[9 + i3]) / d3;
+    sc[c] = f1 * (f2 * f3);
+  }
+  const double denom = 1.0 - (1.0 - qs[2]) * (sc[48] + (sc[50] + sc[56]));
+  for (int c = 0; c < 64; c++) pi[c] = sc[c] / denom;
+}
+
+/* q_p14n (OmegaModel.ml:42-74) as exchangeabilities: q_ij = pi_j s_ij with
+ * s_ij = kappa_part * omega_part for codons one nucleotide apart, else 0. */
+void o_omega_s(const double *qs, double *S) {
+  for (int i = 0; i < 64; i++)
+    for (int j = 0; j < 64; j++) {
+      const int a[3] = {i / 16, (i / 4) % 4, i % 4}, b[3] = {j / 16, (j / 4) % 4, j % 4};
+      int nd = 0, p = -1;
+      for (int k = 0; k < 3; k++)
+        if (a[k] != b[k]) {
+          nd++;
+          p = k;
+        }
+      double s = 0.0;
+      if (nd == 1) {
+        const int x = a[p], y = b[p];
+        /* A<->G, C<->T (A, C, G, T = 0..3) */
+        const int transition = (x == 0 && y == 2) || (x == 2 && y == 0) || (x == 1 && y == 3) ||
+                               (x == 3 && y == 1);
+        const double kp = transition ? qs[0] : 1.0;
+        const double op =
+            (!is_stop_index(i) && !is_stop_index(j) && kAA[i] != kAA[j]) ? qs[1] : 1.0;
+        s = kp * op;
+      }
+      S[i * 64 + j] = s;
+    }
+}
+
+/* update_f3x4 (OmegaModel.ml:102-134): counts from 1 per codon position and
+ * nucleotide over every `Certain leaf; qs9[3 p + n] = count[p][n] / count[p][T]. */
+void o_omega_f3x4(const uint8_t *codes, int64_t ncols, int nl, double *qs9) {
+  long cnt[3][4];
+  for (int p = 0; p < 3; p++)
+    for (int n = 0; n < 4; n++) cnt[p][n] = 1;
+  for (int64_t c = 0; c < ncols; c++)
+    for (int j = 0; j < nl; j++) {
+      const int v = codes[c * nl + j];
+      if (v == 255) continue;
+      cnt[0][v / 16]++;
+      cnt[1][(v / 4) % 4]++;
+      cnt[2][v % 4]++;
+    }
+  for (int p = 0; p < 3; p++)
+    for (int n = 0; n < 3; n++) qs9[3 * p + n] = (double)cnt[p][n] / (double)cnt[p][3];
+}
+
+/* half_cauchy_lpdf ~mode ~scale (OmegaModel.ml:147-154) */
+double o_half_cauchy_lpdf(double mode, double scale, double x) {
+  const double pi = acos(-1.0);
+  const double numer = 1.0 / (pi * scale * (1.0 + pow((x - mode) / scale, 2.0)));
+  const double denom = 1.0 - (atan((0.0 - mode) / scale) / pi + 0.5);
+  return log(numer) - log(denom);
+}
+
+/* lpr_kappa k = log (Gsl.Randist.gamma_pdf ~a:7 ~b:0.25 (k - 1 + epsilon_float))
+ * (OmegaModel.ml:157), with GSL's gamma density gsl_ran_gamma_pdf:
+ * exp((a - 1) log(x / b) - x / b - lngamma(a)) / b for x > 0, a != 1. */
+double o_lpr_kappa(double k) {
+  const double a = 7.0, b = 0.25, x = k - 1.0 + 2.220446049250313e-16;
+  if (x < 0) return -INFINITY;
+  if (x == 0) return -INFINITY;
+  const double p = exp((a - 1.0) * log(x / b) - x / b - lgamma(a)) / b;
+  return log(p);
+}
+
+/* One omega-model instance over a fixed region: Q and its eigensystem for
+ * the current settings (rebuilt when they change) and the tree. */
+typedef struct {
+  int n, nl;
+  const int *lc, *rc;
+  const double *branch;
+  const uint8_t *codes;
+  int64_t ncols;
+  double qs[12];
+  int have_eig;
+  double Q[4096], pi[64], prior[64];
+  o_eig E;
+  int evals;
+  int failed;
+  char msg[256];
+} omega_inst;
+
+static int omega_set_q(omega_inst *I, const double *qs) {
+  if (I->have_eig && memcmp(qs, I->qs, sizeof I->qs) == 0) return 0;
+  double S[4096];
+  memcpy(I->qs, qs, sizeof I->qs);
+  o_omega_s(qs, S);
+  o_omega_pi(qs, I->pi);
+  if (I->have_eig) eig_free(&I->E);
+  I->have_eig = 0;
+  int rc = o_build_q(64, S, I->pi, I->Q, I->msg, sizeof I->msg);
+  if (rc) return rc;
+  rc = eig_of_q(64, I->Q, I->pi, &I->E, I->msg, sizeof I->msg);
+  if (rc) return rc;
+  I->have_eig = 1;
+  /* the root prior: Q.Diag.equilibrium, the stationary distribution, which
+   * for this reversible Q is pi normalised */
+  double tot = 0.0;
+  for (int c = 0; c < 64; c++) tot += I->pi[c];
+  for (int c = 0; c < 64; c++) I->prior[c] = I->pi[c] / tot;
+  return 0;
+}
+
+/* OmegaModel.lpr_leaves (OmegaModel.ml:137-144) at tree scale rho */
+static int omega_lpr(omega_inst *I, double rho, double *lpr_out) {
+  const int n = I->n, nl = I->nl;
+  double *P = (double *)malloc(sizeof(double) * 4096 * (size_t)(n - 1));
+  for (int br = 0; br < n - 1; br++) {
+    int rc = eig_pt(&I->E, rho * I->branch[br], 1e-6, P + (size_t)br * 4096, I->msg,
+                    sizeof I->msg);
+    if (rc) {
+      free(P);
+      return rc;
+    }
+  }
+  o_ws W;
+  ws_init(&W, 64, n, I->lc, I->rc);
+  int leaves[1024];
+  double lpr = 0.;
+  for (int64_t c = 0; c < I->ncols; c++) {
+    for (int j = 0; j < nl; j++) {
+      const uint8_t v = I->codes[c * nl + j];
+      leaves[j] = (v == 255) ? -1 : (int)v;
+    }
+    set_leaves(&W, leaves);
+    lpr = lpr + log(inside(&W, P, I->prior));
+  }
+  ws_free(&W);
+  free(P);
+  *lpr_out = lpr;
+  return 0;
+}
+
+int o_omega_lpr(int n, const int *lc, const int *rc, const double *branch, const double *qs,
+                double rho, const uint8_t *codes, int64_t ncols, double *lpr, char *err,
+                int errlen) {
+  omega_inst I;
+  memset(&I, 0, sizeof I);
+  I.n = n;
+  I.nl = (n + 1) / 2;
+  I.lc = lc;
+  I.rc = rc;
+  I.branch = branch;
+  I.codes = codes;
+  I.ncols = ncols;
+  int r = omega_set_q(&I, qs);
+  if (!r) r = omega_lpr(&I, rho, lpr);
+  if (r) fail(err, errlen, r, "%s", I.msg);
+  if (I.have_eig) eig_free(&I.E);
+  return r;
+}
+
+/* f_rho / f_kappa of kr_map (OmegaModel.ml:160-170) */
+typedef struct {
+  omega_inst *I;
+  int which; /* 0: rho varies (kappa fixed), 1: kappa varies (rho fixed) */
+  double rho, kappa;
+} omega_fn;
+
+static double omega_cb(double x, void *vctx) {
+  omega_fn *F = (omega_fn *)vctx;
+  omega_inst *I = F->I;
+  I->evals++;
+  double qs[12];
+  memcpy(qs, I->qs, sizeof qs);
+  double rho = F->rho, prior_term;
+  if (F->which == 0) {
+    rho = x;
+    prior_term = o_half_cauchy_lpdf(1.0, 0.5, x);
+  } else {
+    qs[0] = x;
+    prior_term = o_lpr_kappa(x);
+  }
+  double l = NAN;
+  if (omega_set_q(I, qs) || omega_lpr(I, rho, &l)) {
+    I->failed = 1;
+    return NAN;
+  }
+  return prior_term + l;
+}
+
+/* PhyloCSFModel.maximize_lpr ~init ~lo ~hi ~accuracy:0.01 (PhyloCSFModel.ml:84-99) */
+static int omega_maximize(omega_fn *F, double init, double lo, double hi, double *x_out,
+                          double *f_out, char *err, int errlen) {
+  int fe = 0;
+  const double good = o_find_init(omega_cb, F, 250, 1, init, lo, hi, &fe);
+  if (F->I->failed) return fail(err, errlen, O_EFAIL, "%s", F->I->msg);
+  double x = good;
+  if (lo < good && good < hi) {
+    int be = 0;
+    const int rc = o_brent_max(omega_cb, F, lo, hi, 0.01, &x, &be, err, errlen);
+    if (F->I->failed) return fail(err, errlen, O_EFAIL, "%s", F->I->msg);
+    if (rc) return rc;
+  }
+  *f_out = omega_cb(x, F);
+  if (F->I->failed) return fail(err, errlen, O_EFAIL, "%s", F->I->msg);
+  *x_out = x;
+  return 0;
+}
+
+/* kr_map (OmegaModel.ml:160-190): three rounds of coordinate ascent, rho in
+ * [0.001, 10] then kappa in [1, 10]; the lpr of the last kappa step
+ * (lpr_kappa + lpr_leaves). */
+static int omega_kr_map(omega_inst *I, double *rho, double *kappa, double *lpr, char *err,
+                        int errlen) {
+  for (int round = 0; round < 3; round++) {
+    omega_fn F = {I, 0, *rho, *kappa};
+    double x, f;
+    int rc = omega_maximize(&F, *rho, 0.001, 10.0, &x, &f, err, errlen);
+    if (rc) return rc;
+    *rho = x;
+    omega_fn G = {I, 1, *rho, *kappa};
+    rc = omega_maximize(&G, *kappa, 1.0, 10.0, &x, &f, err, errlen);
+    if (rc) return rc;
+    *kappa = x;
+    *lpr = f;
+    double qs[12];
+    memcpy(qs, I->qs, sizeof qs);
+    qs[0] = *kappa;
+    if (omega_set_q(I, qs)) return fail(err, errlen, O_EFAIL, "%s", I->msg);
+  }
+  return 0;
+}
+
+/* OmegaModel.score (OmegaModel.ml:195-219).  out[8] = score (decibans),
+ * lpr_H0, lpr_H1, rho_H0, kappa_H0, rho_H1, kappa_H1, evaluations. */
+int o_omega_score(int n, const int *lc, const int *rc, const double *branch,
+                  const uint8_t *codes, int64_t ncols, double omega_H1, double sigma_H1,
+                  double *out, char *err, int errlen) {
+  omega_inst I;
+  memset(&I, 0, sizeof I);
+  I.n = n;
+  I.nl = (n + 1) / 2;
+  I.lc = lc;
+  I.rc = rc;
+  I.branch = branch;
+  I.codes = codes;
+  I.ncols = ncols;
+  double qs[12] = {2.5, 1.0, 1.0};
+  o_omega_f3x4(codes, ncols, I.nl, qs + 3);
+  int r = omega_set_q(&I, qs);
+  if (r) {
+    fail(err, errlen, r, "%s", I.msg);
+    goto done;
+  }
+  double rho0 = 1.0, kappa0 = 2.5, l0 = 0;
+  r = omega_kr_map(&I, &rho0, &kappa0, &l0, err, errlen);
+  if (r) goto done;
+  memcpy(qs, I.qs, sizeof qs);
+  qs[1] = omega_H1;
+  qs[2] = sigma_H1;
+  r = omega_set_q(&I, qs);
+  if (r) {
+    fail(err, errlen, r, "%s", I.msg);
+    goto done;
+  }
+  double rho1 = rho0, kappa1 = kappa0, l1 = 0;
+  r = omega_kr_map(&I, &rho1, &kappa1, &l1, err, errlen);
+  if (r) goto done;
+  out[0] = 10. * (l1 - l0) / log(10.);
+  out[1] = l0;
+  out[2] = l1;
+  out[3] = rho0;
+  out[4] = kappa0;
+  out[5] = rho1;
+  out[6] = kappa1;
+  out[7] = (double)I.evals;
+done:
+  if (I.have_eig) eig_free(&I.E);
+  return r;
 }

@@ -67,6 +67,15 @@ int o_prune(int k, int n, const int *lc, const int *rc, const double *P,
             const double *prior, const int *leaves, int64_t ncols, double *z,
             double *post, const double *dP, double *dlogz_dt);
 
+/* PhyloLik with `Distribution leaves (-2, rows of leaf_dist), node
+ * posteriors of every node and add_branch_posteriors summed over columns with
+ * weights (PhyloLik.ml:9-174, PhyloEM.ml:14-20).  post [ncols][n][k] and
+ * ecounts [(n-1)][k][k] (added to) are optional. */
+int o_lik_posteriors(int k, int n, const int *lc, const int *rc, const double *P,
+                     const double *prior, const int *leaves, const double *leaf_dist,
+                     int64_t ncols, const double *weights, double *z, double *post,
+                     double *ecounts);
+
 typedef struct o_model o_model;
 
 /* PhyloCSFModel.make (src/PhyloCSFModel.ml:107-110) generalised to nm
@@ -94,6 +103,29 @@ int o_score_fixed(o_model *m, const uint8_t *codes, int64_t ncols, double *out,
  * out[7] = score, anc_comp, lpr_c, lpr_n, rho_c, rho_n, n_evals. */
 int o_score_mle(o_model *m, const uint8_t *codes, int64_t ncols, double *out,
                 char *err, int errlen);
+
+/* ---- OmegaModel (src/OmegaModel.ml), in o_model.c ------------------------ */
+
+/* pi_exprs (OmegaModel.ml:24-40) for settings qs[12] (kappa, omega, sigma,
+ * then A, C, G relative to T at codon positions 1, 2, 3): pi[64], not
+ * normalised. */
+void o_omega_pi(const double *qs, double *pi);
+/* exchangeabilities of q_p14n (OmegaModel.ml:42-74): S[64*64] */
+void o_omega_s(const double *qs, double *S);
+/* update_f3x4 (OmegaModel.ml:102-134): qs9 = settings 3..11 */
+void o_omega_f3x4(const uint8_t *codes, int64_t ncols, int nl, double *qs9);
+/* the priors of kr_map (OmegaModel.ml:147-157) */
+double o_half_cauchy_lpdf(double mode, double scale, double x);
+double o_lpr_kappa(double k);
+/* OmegaModel.lpr_leaves (OmegaModel.ml:137-144) */
+int o_omega_lpr(int n, const int *lc, const int *rc, const double *branch, const double *qs,
+                double rho, const uint8_t *codes, int64_t ncols, double *lpr, char *err,
+                int errlen);
+/* OmegaModel.score (OmegaModel.ml:195-219): out[8] = score, lpr_H0, lpr_H1,
+ * rho_H0, kappa_H0, rho_H1, kappa_H1, evaluations */
+int o_omega_score(int n, const int *lc, const int *rc, const double *branch,
+                  const uint8_t *codes, int64_t ncols, double omega_H1, double sigma_H1,
+                  double *out, char *err, int errlen);
 
 /* ---- fit helpers (o_fit.c) --------------------------------------------- */
 

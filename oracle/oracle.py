@@ -53,6 +53,19 @@ def lib():
         L.o_score_mle.argtypes = [C.c_void_p, u8, C.c_int64, dp, C.c_char_p, C.c_int]
         L.o_ocaml_random_floats.argtypes = [C.c_int, C.c_double, C.c_int, dp]
         L.o_md5.argtypes = [C.c_char_p, C.c_uint64, C.c_char_p]
+        L.o_lik_posteriors.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, ip, dp, C.c_int64, dp,
+                                       dp, dp, dp]
+        L.o_omega_pi.argtypes = [dp, dp]
+        L.o_omega_s.argtypes = [dp, dp]
+        L.o_omega_f3x4.argtypes = [u8, C.c_int64, C.c_int, dp]
+        L.o_half_cauchy_lpdf.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.o_half_cauchy_lpdf.restype = C.c_double
+        L.o_lpr_kappa.argtypes = [C.c_double]
+        L.o_lpr_kappa.restype = C.c_double
+        L.o_omega_lpr.argtypes = [C.c_int, ip, ip, dp, dp, C.c_double, u8, C.c_int64, dp,
+                                  C.c_char_p, C.c_int]
+        L.o_omega_score.argtypes = [C.c_int, ip, ip, dp, u8, C.c_int64, C.c_double, C.c_double,
+                                    dp, C.c_char_p, C.c_int]
         _LIB = L
     return _LIB
 
@@ -178,6 +191,31 @@ def prune(tree, P, prior, leaves, posteriors=False, dP=None):
     return z, post, g
 
 
+def lik_posteriors(tree, P, prior, leaves, leaf_dist=None, weights=None, posteriors=True,
+                   ecounts=True):
+    """PhyloLik.prepare with `Certain / `Distribution / `Marginalize leaves
+    (leaves: [ncols, nl] int, -1 marginalise, -2 distribution from
+    leaf_dist [ncols, nl, k]), likelihood, node_posterior of every node
+    and add_branch_posteriors of every branch summed with weights
+    (PhyloEM.collect_sufficient_statistics).
+    -> z [ncols], post [ncols, n, k] or None, ecounts [n-1, k, k] or None."""
+    prior = np.ascontiguousarray(prior, dtype=np.float64)
+    k = len(prior)
+    leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+    ncols = leaves.shape[0]
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    ld = None if leaf_dist is None else np.ascontiguousarray(leaf_dist, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    z = np.zeros(ncols)
+    post = np.zeros((ncols, tree.n, k)) if posteriors else None
+    ec = np.zeros((tree.n - 1, k, k)) if ecounts else None
+    lib().o_lik_posteriors(k, tree.n, _ip(tree.lc.astype(np.int32)), _ip(tree.rc.astype(np.int32)),
+                           _dp(P), _dp(prior), _ip(leaves), None if ld is None else _dp(ld), ncols,
+                           None if w is None else _dp(w), _dp(z),
+                           None if post is None else _dp(post), None if ec is None else _dp(ec))
+    return z, post, ec
+
+
 class Model:
     """PhyloCSFModel.make over one tree and nm ECMs (0 = coding, 1 = noncoding)."""
 
@@ -251,3 +289,62 @@ def md5(data: bytes) -> bytes:
     out = C.create_string_buffer(16)
     lib().o_md5(data, len(data), out)
     return out.raw
+
+
+# ---- OmegaModel (src/OmegaModel.ml) -----------------------------------------
+
+def omega_pi(qs):
+    """pi_exprs (OmegaModel.ml:24-40) for the 12 settings, not normalised."""
+    qs = np.ascontiguousarray(qs, dtype=np.float64)
+    out = np.zeros(64)
+    lib().o_omega_pi(_dp(qs), _dp(out))
+    return out
+
+
+def omega_s(qs):
+    """exchangeabilities s (q_ij = pi_j s_ij) of q_p14n (OmegaModel.ml:42-74)."""
+    qs = np.ascontiguousarray(qs, dtype=np.float64)
+    out = np.zeros((64, 64))
+    lib().o_omega_s(_dp(qs), _dp(out))
+    return out
+
+
+def omega_f3x4(codes):
+    """update_f3x4 (OmegaModel.ml:102-134): settings 3..11 from codes [ncols, nl]."""
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    out = np.zeros(9)
+    lib().o_omega_f3x4(_u8(c), c.shape[0], c.shape[1], _dp(out))
+    return out
+
+
+def half_cauchy_lpdf(mode, scale, x):
+    return lib().o_half_cauchy_lpdf(float(mode), float(scale), float(x))
+
+
+def lpr_kappa(k):
+    return lib().o_lpr_kappa(float(k))
+
+
+def omega_lpr(tree, qs, rho, codes):
+    """OmegaModel.lpr_leaves (OmegaModel.ml:137-144)."""
+    qs = np.ascontiguousarray(qs, dtype=np.float64)
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    out = C.c_double(0)
+    err = C.create_string_buffer(512)
+    _check(lib().o_omega_lpr(tree.n, _ip(tree.lc.astype(np.int32)), _ip(tree.rc.astype(np.int32)),
+                             _dp(tree.branch.astype(np.float64)), _dp(qs), float(rho), _u8(c),
+                             c.shape[0], C.byref(out), err, 512), err)
+    return out.value
+
+
+def omega_score(tree, codes, omega_H1=0.2, sigma_H1=0.01):
+    """OmegaModel.score (OmegaModel.ml:195-219) -> dict."""
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    out = np.zeros(8)
+    err = C.create_string_buffer(512)
+    _check(lib().o_omega_score(tree.n, _ip(tree.lc.astype(np.int32)),
+                               _ip(tree.rc.astype(np.int32)), _dp(tree.branch.astype(np.float64)),
+                               _u8(c), c.shape[0], float(omega_H1), float(sigma_H1), _dp(out),
+                               err, 512), err)
+    return dict(score=out[0], lpr_H0=out[1], lpr_H1=out[2], rho_H0=out[3], kappa_H0=out[4],
+                rho_H1=out[5], kappa_H1=out[6], evals=int(out[7]))

@@ -33,7 +33,8 @@ EXPORTED = [
     "pcsf_backward", "pcsf_batch_device_results", "pcsf_batch_last_timing",
     "pcsf_lik_columns", "pcsf_simulate", "pcsf_autotune", "pcsf_set_variant", "pcsf_score",
     "pcsf_model_specialized", "pcsf_batch_create_frames", "pcsf_simulate_nucleotides",
-    "pcsf_program", "pcsf_batch_codes", "pcsf_forward_noalloc",
+    "pcsf_program", "pcsf_batch_codes", "pcsf_forward_noalloc", "pcsf_score_omega",
+    "pcsf_lik_posteriors",
 ]
 PCSF_STRATEGY_FIXED = 0
 PCSF_STRATEGY_MLE = 1
@@ -113,6 +114,8 @@ def lib():
     L.pcsf_batch_device_results.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
     L.pcsf_batch_last_timing.argtypes = [vp, dp, dp, dp]
     L.pcsf_lik_columns.argtypes = [vp, C.c_int, C.c_int, i32p, dp, dp, i32p, C.c_int64, dp, dp]
+    L.pcsf_lik_posteriors.argtypes = [vp, C.c_int, C.c_int, i32p, dp, dp, i32p, dp, C.c_int64,
+                                      dp, dp, dp, dp]
     L.pcsf_simulate.argtypes = [vp, C.c_int32, i64p, i32p, dp, C.c_int, C.c_double,
                                 C.c_double, C.c_double, C.c_uint64, C.c_int, u8p]
     L.pcsf_simulate_nucleotides.argtypes = L.pcsf_simulate.argtypes
@@ -120,6 +123,8 @@ def lib():
     L.pcsf_set_variant.argtypes = [vp, C.c_int, C.c_int]
     L.pcsf_model_specialized.argtypes = [vp, C.c_int, ip]
     L.pcsf_score.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, dp, dp, i32p, i32p]
+    L.pcsf_score_omega.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, i32p,
+                                   i32p]
     L.pcsf_program.argtypes = [C.c_int, i32p, C.c_int, i32p, C.c_int, ip, ip]
     _lib = L
     return L

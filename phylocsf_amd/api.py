@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -183,6 +183,24 @@ class PhyloCSFModel:
         self.ctx.check(lib().pcsf_model_create(self.ctx.h, tree.n, _i32(ch), _dp(br), self.nm,
                                                _dp(S), _dp(pi), C.byref(h)))
         self.h = h
+
+    @classmethod
+    def tree_only(cls, ctx: Context, tree: Tree):
+        """A tree without rate models, for the omega test (pcsf_model_create
+        with n_models = 0)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.tree = tree
+        self.nm = 2
+        self.pi = None
+        ch = np.ascontiguousarray(tree.children, dtype=np.int32)
+        br = np.ascontiguousarray(tree.branch, dtype=np.float64).copy()
+        br[-1] = 0.0
+        h = C.c_void_p()
+        ctx.check(lib().pcsf_model_create(ctx.h, tree.n, _i32(ch), _dp(br), 0, None, None,
+                                          C.byref(h)))
+        self.h = h
+        return self
 
     @classmethod
     def from_paramset(cls, ctx: Optional[Context], paramset: str, species=None):
@@ -399,6 +417,22 @@ def score(batch: Batch, strategy="fixed", precision=PCSF_FP32):
     return out
 
 
+def score_omega(batch: Batch, omega_H1=0.2, sigma_H1=0.01, precision=PCSF_FP64):
+    """OmegaModel.score for every region of a batch (src/OmegaModel.ml:195-219):
+    -> dict of arrays score (decibans), lpr, rho, kappa ([R, 2], H0 first),
+    status, n_evals."""
+    R = batch.n_regions
+    out = dict(score=np.zeros(R), lpr=np.zeros((R, 2)), rho=np.zeros((R, 2)),
+               kappa=np.zeros((R, 2)), status=np.zeros(R, np.int32),
+               n_evals=np.zeros(R, np.int32))
+    batch.model.ctx.check(lib().pcsf_score_omega(batch.h, float(omega_H1), float(sigma_H1),
+                                                 int(precision), _dp(out["score"]),
+                                                 _dp(out["lpr"]), _dp(out["rho"]),
+                                                 _dp(out["kappa"]), _i32(out["status"]),
+                                                 _i32(out["n_evals"])))
+    return out
+
+
 def score_fixed(batch: Batch, precision=PCSF_FP32):
     """llr_FixedLik 1.0 per region (src/PhyloCSFModel.ml:122-128):
     -> dict of arrays score, anc_comp, lpr_c, lpr_n (decibans for the first two)."""
@@ -409,12 +443,17 @@ def score_fixed(batch: Batch, precision=PCSF_FP32):
 
 class PhyloLik:
     """CamlPaml.PhyloLik over many columns at once (lib/CamlPaml/PhyloLik.mli):
-    prepare(tree, pms, prior, leaves) then likelihood / node_posterior(root)."""
+    prepare(tree, pms, prior, leaves) then likelihood / node_posterior /
+    branch_posteriors / add_branch_posteriors of the intermediate.
+
+    leaves: [ncols, nl] int -- a state index (`Certain), -1 (`Marginalize) or
+    -2 (`Distribution, its probabilities in leaf_dist[col, leaf, :])."""
 
     def __init__(self, ctx: Context):
         self.ctx = ctx
 
-    def prepare(self, tree: Tree, pms: np.ndarray, prior: np.ndarray, leaves: np.ndarray):
+    def prepare(self, tree: Tree, pms: np.ndarray, prior: np.ndarray, leaves: np.ndarray,
+                leaf_dist: Optional[np.ndarray] = None):
         k = len(prior)
         leaves = np.ascontiguousarray(leaves, dtype=np.int32)
         if leaves.ndim == 1:
@@ -423,27 +462,97 @@ class PhyloLik:
             raise InvalidArgument(-1, "CamlPaml.Infer.prepare: length(leaves) != leaves(t)")
         if pms.shape[0] < tree.n - 1:
             raise InvalidArgument(-1, "CamlPaml.Infer.prepare: not enough P matrices")
+        if (leaves == -2).any() and leaf_dist is None:
+            raise InvalidArgument(-1, "CamlPaml.Infer.prepare: `Distribution leaves need leaf_dist")
         P = np.ascontiguousarray(pms[:tree.n - 1], dtype=np.float64)
         pr = np.ascontiguousarray(prior, dtype=np.float64)
+        ld = None if leaf_dist is None else np.ascontiguousarray(leaf_dist, dtype=np.float64)
+        if ld is not None and ld.shape != (leaves.shape[0], tree.nl, k):
+            raise InvalidArgument(-1, "CamlPaml.Infer.prepare: leaf_dist must be [ncols, nl, k]")
         ncols = leaves.shape[0]
         z = np.zeros(ncols)
         post = np.zeros((ncols, k))
         ch = np.ascontiguousarray(tree.children, dtype=np.int32)
-        self.ctx.check(lib().pcsf_lik_columns(self.ctx.h, k, tree.n, _i32(ch), _dp(P), _dp(pr),
-                                              _i32(leaves), ncols, _dp(z), _dp(post)))
-        return _Intermediate(tree, z, post)
+        if ld is None:
+            self.ctx.check(lib().pcsf_lik_columns(self.ctx.h, k, tree.n, _i32(ch), _dp(P),
+                                                  _dp(pr), _i32(leaves), ncols, _dp(z),
+                                                  _dp(post)))
+        else:
+            allpost = np.zeros((ncols, tree.n, k))
+            self.ctx.check(lib().pcsf_lik_posteriors(self.ctx.h, k, tree.n, _i32(ch), _dp(P),
+                                                     _dp(pr), _i32(leaves), _dp(ld), ncols,
+                                                     None, _dp(z), _dp(allpost), None))
+            post = allpost[:, -1, :].copy()
+        it = _Intermediate(self.ctx, tree, P, pr, leaves, ld, z, post)
+        if ld is not None:
+            it._all = allpost
+        return it
 
 
-@dataclass
 class _Intermediate:
-    tree: Tree
-    z: np.ndarray
-    root_post: np.ndarray
+    """The result of PhyloLik.prepare for many columns (one reference
+    intermediate per column)."""
+
+    def __init__(self, ctx, tree, P, prior, leaves, leaf_dist, z, root_post):
+        self.ctx, self.tree, self.P, self.prior = ctx, tree, P, prior
+        self.leaves, self.leaf_dist, self.z, self.root_post = leaves, leaf_dist, z, root_post
+        self._all = None
+
+    def _call(self, node_post, ecounts, weights):
+        k = len(self.prior)
+        ch = np.ascontiguousarray(self.tree.children, dtype=np.int32)
+        z = np.zeros(self.leaves.shape[0])
+        w = None if weights is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(weights, np.float64), z.shape))
+        self.ctx.check(lib().pcsf_lik_posteriors(
+            self.ctx.h, k, self.tree.n, _i32(ch), _dp(self.P), _dp(self.prior),
+            _i32(self.leaves), None if self.leaf_dist is None else _dp(self.leaf_dist),
+            len(z), None if w is None else _dp(w), _dp(z),
+            None if node_post is None else _dp(node_post),
+            None if ecounts is None else _dp(ecounts)))
 
     def likelihood(self):
+        """PhyloLik.likelihood: z per column."""
         return self.z
 
     def node_posterior(self, i):
-        if i != self.tree.n - 1:
-            raise InvalidArgument(-1, "node_posterior: only the root is served by this entry")
-        return self.root_post
+        """PhyloLik.node_posterior (PhyloLik.ml:127-138): [ncols, k]."""
+        if not 0 <= i < self.tree.n:
+            raise InvalidArgument(-1, "node_posterior: no such node")
+        if i == self.tree.n - 1:
+            return self.root_post
+        if self._all is None:
+            self._all = np.zeros((self.leaves.shape[0], self.tree.n, len(self.prior)))
+            self._call(self._all, None, None)
+        return self._all[:, i, :]
+
+    def add_branch_posteriors(self, ecounts, weight=1.0):
+        """PhyloLik.add_branch_posteriors (PhyloLik.ml:140-174) of every branch,
+        summed over the columns: ecounts [n-1, k, k] += weight * P(parent = a,
+        node = b | leaves); weight a scalar or one per column."""
+        if ecounts.shape != (self.tree.n - 1, len(self.prior), len(self.prior)) or \
+                ecounts.dtype != np.float64 or not ecounts.flags.c_contiguous:
+            raise InvalidArgument(-1, "add_branch_posteriors: ecounts must be a C-contiguous "
+                                      "float64 [n-1, k, k] array")
+        self._call(None, ecounts, weight)
+        return ecounts
+
+    def branch_posteriors(self, br):
+        """PhyloLik.branch_posteriors (PhyloLik.mli:31-32), summed over the
+        columns: [k, k]."""
+        if not 0 <= br < self.tree.n - 1:
+            raise InvalidArgument(-1, "CamlPaml.Infer.add_branch_posteriors: invalid branch")
+        k = len(self.prior)
+        ec = np.zeros((self.tree.n - 1, k, k))
+        self.add_branch_posteriors(ec)
+        return ec[br]
+
+
+def collect_sufficient_statistics(ctx: Context, tree: Tree, pms, prior, leaves, ss,
+                                  leaf_dist=None, weights=None):
+    """PhyloEM.collect_sufficient_statistics (lib/CamlPaml/PhyloEM.ml:14-20) over
+    many columns: adds every column's branch posteriors (times its weight) to
+    ss [n-1, k, k] and returns the likelihood of each column."""
+    it = PhyloLik(ctx).prepare(tree, pms, prior, leaves, leaf_dist)
+    it.add_branch_posteriors(ss, 1.0 if weights is None else weights)
+    return it.likelihood()

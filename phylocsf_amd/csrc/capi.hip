@@ -8,8 +8,10 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <thread>
 
 #include "host/maximize.hpp"
+#include "host/omega.hpp"
 #include "host/program.hpp"
 #include "host/text.hpp"
 #include "internal.hpp"
@@ -54,10 +56,13 @@ void dfree(T*& p) {
 
 bool rho_valid(double r) { return r > 0.0 && std::isfinite(r); }
 
+constexpr const char* kTreeOnly =
+    "a tree-only model (made with n_models = 0) serves the omega test only";
+
 void free_psets(PSets& ps);
 
 hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0,
-                       bool tangent = false) {
+                       bool tangent = false, bool own_models = false) {
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) {
     if (e == hipSuccess) e = x;
@@ -76,6 +81,13 @@ hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0,
   if (nctab > 0) {
     chk(dalloc(&ps.d_ctab, (size_t)cap * nctab * kTabCodes * kTabCodes * 64));
     chk(dalloc(&ps.d_clnt, (size_t)cap * nctab * kTabCodes * kTabCodes));
+  }
+  if (own_models) {
+    chk(dalloc(&ps.d_eVT, (size_t)cap * 4096));
+    chk(dalloc(&ps.d_eVinv, (size_t)cap * 4096));
+    chk(dalloc(&ps.d_elam, (size_t)cap * 64));
+    chk(dalloc(&ps.d_epi, (size_t)cap * 64));
+    chk(dalloc(&ps.d_elnpi, (size_t)cap * 64));
   }
   if (e != hipSuccess) {
     free_psets(ps);  // nothing half-allocated survives: cap stays 0
@@ -98,6 +110,11 @@ void free_psets(PSets& ps) {
   dfree(ps.d_clnt);
   dfree(ps.d_DT32);
   dfree(ps.d_Dfrag);
+  dfree(ps.d_eVT);
+  dfree(ps.d_eVinv);
+  dfree(ps.d_elam);
+  dfree(ps.d_epi);
+  dfree(ps.d_elnpi);
   ps.cap = 0;
   ps.fp64 = false;
   ps.fmt = -1;
@@ -109,7 +126,8 @@ void free_psets(PSets& ps) {
 // (Q.Diag.real_to_Pt, lib/CamlPaml/Q.ml:211-249), which also gives the
 // offending value; should the host copy pass where the device's failed (a
 // rounding-level difference), the device's position is reported.
-int pbuild_failure(pcsf_model* m, unsigned long long key, const double* rho_host) {
+int pbuild_failure(pcsf_model* m, unsigned long long key, const double* rho_host,
+                   bool own_models) {
   pcsf_ctx* c = m->ctx;
   const int check = (int)(key & 3);
   unsigned long long pos = key >> 2;
@@ -123,7 +141,7 @@ int pbuild_failure(pcsf_model* m, unsigned long long key, const double* rho_host
   if (check == 0) return set_err(c, PCSF_EINVAL, "CamlPaml.Q.to_Pt");
   std::vector<double> P(4096);
   std::string msg;
-  if (!pcsf::transition_host(m->eig[set % m->nm], t, 1e-6, P.data(), &msg)) {
+  if (!own_models && !pcsf::transition_host(m->eig[set % m->nm], t, 1e-6, P.data(), &msg)) {
     if (msg.rfind("Assert_failure", 0) == 0) msg = "Assert_failure (Q.ml:245)";
     return set_err(c, PCSF_EFAIL, msg);
   }
@@ -166,7 +184,7 @@ int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
-  if (key != ~0ull) return pbuild_failure(m, key, rho_host);
+  if (key != ~0ull) return pbuild_failure(m, key, rho_host, ps.d_eVT != nullptr);
   ps.fmt = fmt;
   return PCSF_OK;
 }
@@ -245,16 +263,28 @@ int ensure_aligned(pcsf_batch* b) {
 }
 
 // Bytes of one transition set on the device.
-size_t set_bytes(const pcsf_model* m, bool fp64, bool tangent = false) {
+size_t set_bytes(const pcsf_model* m, bool fp64, bool tangent = false, bool own_model = false) {
   return (size_t)m->nbr * (kLeafRows * 64 * 4 + kTileFloats * 4 + kLeafRows * 8 +
-                           (fp64 ? 4096 * 8 : 0) + (tangent ? (kLeafRows + 64) * 64 * 4 : 0));
+                           (fp64 ? 4096 * 8 : 0) + (tangent ? (kLeafRows + 64) * 64 * 4 : 0)) +
+         (own_model ? (2 * 4096 + 3 * 64) * 8 : 0);
 }
+
+// Rate models of their own for every (region, model) pair k = r * nm + q
+// (host arrays): V transposed, V^-1, eigenvalues, root prior and its log.
+struct SetModels {
+  const double* VT;     // np * 4096
+  const double* Vinv;   // np * 4096
+  const double* lam;    // np * 64
+  const double* pi;     // np * 64
+  const double* lnpi;   // np * 64
+};
 
 // Forward with one tree scale per (region, model): rho[r*nm+q].  active
 // (optional, r*nm+q) restricts the work to the listed pairs; results of the
 // others are left as they were.  Regions go through in chunks whose
 // transition sets fit the memory budget.
-int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int precision) {
+int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int precision,
+                    const SetModels* sm = nullptr) {
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
   const int nm = m->nm;
@@ -263,11 +293,12 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
   const bool fp64 = precision == PCSF_FP64;
   const int fmt = fp64 ? kFragF32 : frag_format(precision);
   const size_t budget = (size_t)8 << 30;
-  int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, fp64) / nm * nm);
+  const bool own = sm != nullptr;
+  int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, fp64, false, own) / nm * nm);
   cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
-  if (b->rs.cap < cap || (fp64 && !b->rs.fp64)) {
+  if (b->rs.cap < cap || (fp64 && !b->rs.fp64) || (own != (b->rs.d_eVT != nullptr))) {
     free_psets(b->rs);
-    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, fp64));
+    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, fp64, 0, false, own));
   }
   const int chunk_regions = b->rs.cap / nm;
   int variant = fp64 ? 0 : m->mfma_variant[precision];
@@ -295,6 +326,18 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
                               hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(b->rs.d_active, act.data(), act.size(), hipMemcpyHostToDevice,
                               c->stream));
+    if (own) {  // the chunk's rate models
+      const size_t k0 = (size_t)r0 * nm, nk = (size_t)nr * nm;
+      auto up = [&](double* dst, const double* src, size_t per) {
+        return hipMemcpyAsync(dst, src + k0 * per, nk * per * 8, hipMemcpyHostToDevice,
+                              c->stream);
+      };
+      HIP_TRY(c, up(b->rs.d_eVT, sm->VT, 4096));
+      HIP_TRY(c, up(b->rs.d_eVinv, sm->Vinv, 4096));
+      HIP_TRY(c, up(b->rs.d_elam, sm->lam, 64));
+      HIP_TRY(c, up(b->rs.d_epi, sm->pi, 64));
+      HIP_TRY(c, up(b->rs.d_elnpi, sm->lnpi, 64));
+    }
     float tp = 0.f;
     rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, fmt, &tp);
     if (rc) {
@@ -564,8 +607,14 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
                       int nm, const double* S, const double* pi, pcsf_model** out) {
   if (!out) return PCSF_EINVAL;
   *out = nullptr;
-  if (n < 3 || n % 2 == 0 || !children || !branch || nm < 1 || nm > 8 || !S || !pi)
+  if (n < 3 || n % 2 == 0 || !children || !branch || nm < 0 || nm > 8 ||
+      (nm > 0 && (!S || !pi)))
     return set_err(c, PCSF_EINVAL, "CamlPaml.PhyloModel.make: bad tree or model count");
+  // no rate models: a tree for models carried per set (the omega test's two
+  // hypotheses per region)
+  const bool tree_only = nm == 0;
+  const int nrm = nm;  // rate models given
+  if (tree_only) nm = 2;
   const int nl = (n + 1) / 2;
   if (n - 1 >= kNoBranch) return set_err(c, PCSF_EINVAL, "tree too large");
   auto m = new (std::nothrow) pcsf_model();
@@ -576,6 +625,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   m->nl = nl;
   m->nm = nm;
   m->nbr = n - 1;
+  m->tree_only = tree_only;
   m->lc.assign(n, -1);
   m->rc.assign(n, -1);
   m->parent.assign(n, -1);
@@ -598,7 +648,7 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   // an ECM's symmetric exchangeabilities give; a non-symmetric S would need
   // the reference's complex path (lib/CamlPaml/Q.ml:124-141,218-223), which
   // this build does not carry, so it is refused rather than symmetrised.
-  for (int q = 0; q < nm; ++q)
+  for (int q = 0; q < nrm; ++q)
     for (int i = 0; i < 64; ++i)
       for (int j = 0; j < i; ++j) {
         const double a = S[q * 4096 + i * 64 + j], b = S[q * 4096 + j * 64 + i];
@@ -611,11 +661,12 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
           return set_err(c, PCSF_EINVAL, buf);
         }
       }
-  m->eig.resize(nm);
+  m->eig.resize(nrm);
   m->cached_rho.reserve(nm);
-  m->pi.assign(pi, pi + 64 * nm);
-  std::vector<double> V(nm * 4096), Vi(nm * 4096), lam(nm * 64), lnpi(nm * 64), QT(nm * 4096);
-  for (int q = 0; q < nm; ++q) {
+  if (nrm) m->pi.assign(pi, pi + 64 * nrm);
+  std::vector<double> V(nrm * 4096), Vi(nrm * 4096), lam(nrm * 64), lnpi(nrm * 64),
+      QT(nrm * 4096);
+  for (int q = 0; q < nrm; ++q) {
     std::vector<double> Q(4096);
     std::string msg;
     if (!pcsf::build_rate_matrix(64, S + q * 4096, pi + q * 64, Q.data(), &msg))
@@ -686,10 +737,11 @@ int pcsf_model_create(pcsf_ctx* c, int n, const int32_t* children, const double*
   HIP_TRY(c, dalloc(&m->d_lsl, std::max<size_t>(1, lsl.size())));
   HIP_TRY(c, dalloc(&m->d_lsl_t, std::max<size_t>(1, lsl_t.size())));
   HIP_TRY(c, dalloc(&m->d_perr, (size_t)1));
-  HIP_TRY(c, alloc_psets(m->fixed, nm, m->nbr, true, (int)m->cherries.size() / 3));
+  if (!tree_only)
+    HIP_TRY(c, alloc_psets(m->fixed, nm, m->nbr, true, (int)m->cherries.size() / 3));
   std::vector<int32_t> lc32(m->lc.begin(), m->lc.end()), rc32(m->rc.begin(), m->rc.end());
   auto up = [&](void* dst, const void* src, size_t bytes) {
-    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
   };
   HIP_TRY(c, up(m->d_VT, V.data(), V.size() * 8));
   HIP_TRY(c, up(m->d_Vinv, Vi.data(), Vi.size() * 8));
@@ -743,6 +795,7 @@ int pcsf_model_info(const pcsf_model* m, int* n, int* nl, int* nm, int* slots) {
 int pcsf_model_transition(pcsf_model* m, int model, double rho, double* P_out) {
   if (!m || !P_out || !m->ctx) return PCSF_EINVAL;
   pcsf_ctx* c = m->ctx;
+  if (m->tree_only) return set_err(c, PCSF_EINVAL, kTreeOnly);
   if (model < 0 || model >= m->nm) return set_err(c, PCSF_EINVAL, "model index out of range");
   if (!rho_valid(rho))
     return set_err(c, PCSF_EINVAL, "CamlPaml.P14n.instantiate_tree: domain violation on variable 0");
@@ -926,6 +979,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   if (!b || !rho) return PCSF_EINVAL;
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
+  if (m->tree_only) return set_err(c, PCSF_EINVAL, kTreeOnly);
   (void)save_for_backward;
   if (precision < PCSF_FP64 || precision > PCSF_BF16)
     return set_err(c, PCSF_EINVAL, "pcsf_forward: unknown precision");
@@ -996,6 +1050,7 @@ int pcsf_forward_noalloc(pcsf_model* m, const uint8_t* d_codes, int64_t n_cols,
                          int precision, double* d_loglik, double* d_elpr) {
   if (!m || !m->ctx || !rho) return PCSF_EINVAL;
   pcsf_ctx* c = m->ctx;
+  if (m->tree_only) return set_err(c, PCSF_EINVAL, kTreeOnly);
   if (precision < PCSF_FP64 || precision > PCSF_BF16)
     return set_err(c, PCSF_EINVAL, "pcsf_forward_noalloc: unknown precision");
   if (n_cols < 0 || n_regions < 0 || (n_cols > 0 && !d_codes) || !d_region_offsets ||
@@ -1089,6 +1144,7 @@ int pcsf_autotune(pcsf_batch* b, int precision, int reps, double* ms_out, int* c
   if (!b || reps < 1) return PCSF_EINVAL;
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
+  if (m->tree_only) return set_err(c, PCSF_EINVAL, kTreeOnly);
   if (precision == PCSF_FP64 || frag_format(precision) < 0)
     return set_err(c, PCSF_EINVAL, "pcsf_autotune: not an MFMA precision");
   HIP_TRY(c, hipSetDevice(c->device));
@@ -1335,6 +1391,24 @@ int score_mle(pcsf_batch* b, int precision, std::vector<PairState>& st) {
       }
     }
   }
+  if (precision != PCSF_FP64) {
+    // the reference's final `f x` in FP64: the search ran on the MFMA
+    // precision's likelihoods, the score is FP64's at its estimate
+    for (size_t k = 0; k < np; ++k) {
+      act[k] = st[k].status == PCSF_OK;
+      rho[k] = act[k] ? st[k].res_x : 1.0;
+    }
+    rc = forward_regions(b, rho.data(), act.data(), PCSF_FP64);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(E.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < np; ++k)
+      if (act[k]) {
+        st[k].res_lpr = L[k];
+        st[k].res_elpr = E[k];
+        st[k].evals++;
+      }
+  }
   return PCSF_OK;
 }
 
@@ -1458,10 +1532,12 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
       s.x = xn;
     }
   }
-  // lpr and elpr at the estimates, in the requested precision
+  // lpr and elpr at the estimates, in FP64 whatever the search ran in: the
+  // MFMA precisions' rounding (up to ~6e-9 of lpr on 100vertebrates regions
+  // of a few thousand codons, ~3e-3 dB) would otherwise reach the score
   for (size_t k = 0; k < np; ++k) rho[k] = std::exp(g[k].x);
   std::vector<uint8_t> all(np, 1);
-  int rc = forward_regions(b, rho.data(), all.data(), precision);
+  int rc = forward_regions(b, rho.data(), all.data(), PCSF_FP64);
   if (rc) return rc;
   HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
   HIP_TRY(c, hipMemcpy(D.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost));
@@ -1474,6 +1550,193 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   return PCSF_OK;
 }
 
+// The omega test (src/OmegaModel.ml:160-219) for every region of a batch:
+// per region, H0 (omega = sigma = 1) and then H1 (omega_H1, sigma_H1, from
+// H0's estimates) are each fitted by kr_map, three rounds of coordinate
+// ascent -- maximize_lpr over the tree scale in [0.001, 10], then over kappa
+// in [1, 10] -- of the log prior plus lpr.  Every (region, hypothesis) pair
+// is a state machine; all pairs advance in lock step, one batched
+// per-region forward per step, with the rate model of each pair's current
+// settings as the set's own (its eigensystem made on the host when kappa
+// changes).
+struct OmegaPair {
+  double qs[pcsf::kOmegaSettings];
+  double rho = 1.0, kappa = 2.5;
+  int round = 0, phase = 0;  // phase 0: tree scale, 1: kappa, 2: done
+  bool waiting = false;      // H1 before its H0 is done
+  pcsf::MaximizeLpr mx;
+  double lpr = 0;
+  int status = PCSF_OK;
+  int evals = 0;
+  double eig_kappa = -1;     // kappa of the eigensystem in the host arrays
+};
+
+int score_omega(pcsf_batch* b, int precision, double omega_h1, double sigma_h1,
+                std::vector<OmegaPair>& st) {
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  const int R = b->nreg;
+  const size_t np = (size_t)R * 2;
+  const double rel_tol = precision == PCSF_FP64 ? 1.4901161193847656e-08 : 1e-5;
+  // F3x4 settings per region, from the batch's codes
+  std::vector<uint8_t> codes((size_t)b->ncols * m->nl);
+  int rc = pcsf_batch_codes(b, codes.data(), nullptr);
+  if (rc) return rc;
+  st.assign(np, OmegaPair());
+  for (int r = 0; r < R; ++r) {
+    double f3[9];
+    pcsf::omega_f3x4(codes.data() + (size_t)b->offsets[r] * m->nl,
+                     b->offsets[r + 1] - b->offsets[r], m->nl, f3);
+    for (int h = 0; h < 2; ++h) {
+      OmegaPair& p = st[(size_t)r * 2 + h];
+      p.qs[0] = 2.5;  // new_instance ~kappa:2.5 (OmegaModel.ml:197)
+      p.qs[1] = 1.0;
+      p.qs[2] = 1.0;
+      std::copy(f3, f3 + 9, p.qs + 3);
+      p.waiting = h == 1;
+      if (h == 0) p.mx.start(p.rho, 0.001, 10.0, 0.01, rel_tol);
+    }
+  }
+  std::vector<double> VT(np * 4096), Vi(np * 4096), lam(np * 64), pri(np * 64), lnpri(np * 64);
+  std::vector<double> rho(np, 1.0), L(np);
+  std::vector<uint8_t> act(np);
+  SetModels sm{VT.data(), Vi.data(), lam.data(), pri.data(), lnpri.data()};
+  // the rate model of pair k at kappa x: eigensystem and root prior
+  auto build = [&](size_t k, double x) -> bool {
+    OmegaPair& p = st[k];
+    double qs[pcsf::kOmegaSettings];
+    std::copy(p.qs, p.qs + pcsf::kOmegaSettings, qs);
+    qs[0] = x;
+    std::vector<double> S(4096), Q(4096);
+    double pi[64];
+    pcsf::omega_exchangeabilities(qs, S.data());
+    pcsf::omega_pi(qs, pi);
+    pcsf::Eigen e;
+    if (!pcsf::build_rate_matrix(64, S.data(), pi, Q.data(), nullptr) ||
+        !pcsf::eigen_reversible(64, Q.data(), pi, &e, nullptr))
+      return false;
+    // Q.Diag.equilibrium: the stationary distribution, pi normalised
+    double tot = 0.0;
+    for (double v : pi) tot += v;
+    for (int i = 0; i < 64; ++i) {
+      for (int j = 0; j < 64; ++j) VT[k * 4096 + j * 64 + i] = e.V[i * 64 + j];
+      pri[k * 64 + i] = pi[i] / tot;
+      lnpri[k * 64 + i] = std::log(pi[i] / tot);
+    }
+    std::copy(e.Vinv.begin(), e.Vinv.end(), Vi.begin() + k * 4096);
+    std::copy(e.lam.begin(), e.lam.end(), lam.begin() + k * 64);
+    p.eig_kappa = x;
+    return true;
+  };
+  const int nthreads =
+      (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  for (int step = 0; step < 1000000; ++step) {
+    bool any = false;
+    std::vector<size_t> need;
+    for (size_t k = 0; k < np; ++k) {
+      OmegaPair& p = st[k];
+      act[k] = p.phase < 2 && !p.waiting && p.status == PCSF_OK;
+      if (!act[k]) continue;
+      any = true;
+      const double x = p.mx.pending();
+      rho[k] = p.phase == 0 ? x : p.rho;
+      const double kap = p.phase == 1 ? x : p.kappa;
+      if (kap != p.eig_kappa) need.push_back(k);
+    }
+    if (!any) break;
+    // new eigensystems on host threads
+    {
+      std::vector<std::thread> pool;
+      std::vector<char> ok(need.size(), 1);
+      for (int t = 0; t < nthreads; ++t)
+        pool.emplace_back([&, t]() {
+          for (size_t i = t; i < need.size(); i += nthreads) {
+            const size_t k = need[i];
+            const double kap = st[k].phase == 1 ? st[k].mx.pending() : st[k].kappa;
+            ok[i] = build(k, kap) ? 1 : 0;
+          }
+        });
+      for (auto& th : pool) th.join();
+      for (size_t i = 0; i < need.size(); ++i)
+        if (!ok[i]) {
+          st[need[i]].status = PCSF_EFAIL;
+          act[need[i]] = 0;
+        }
+    }
+    rc = forward_regions(b, rho.data(), act.data(), precision, &sm);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < np; ++k) {
+      if (!act[k]) continue;
+      OmegaPair& p = st[k];
+      const double x = p.mx.pending();
+      // f_rho / f_kappa (OmegaModel.ml:161-170): log prior + lpr
+      const double prior = p.phase == 0 ? pcsf::omega_lpr_rho(x) : pcsf::omega_lpr_kappa(x);
+      p.evals++;
+      p.mx.feed(prior + L[k]);
+      if (!p.mx.done()) continue;
+      if (p.mx.failed) {
+        p.status = PCSF_EFAIL;
+        continue;
+      }
+      if (p.phase == 0) {
+        p.rho = p.mx.x;
+        p.phase = 1;
+        p.mx.start(p.kappa, 1.0, 10.0, 0.01, rel_tol);
+        continue;
+      }
+      p.kappa = p.mx.x;
+      p.qs[0] = p.kappa;
+      p.lpr = p.mx.fx;
+      if (++p.round < 3) {
+        p.phase = 0;
+        p.mx.start(p.rho, 0.001, 10.0, 0.01, rel_tol);
+        continue;
+      }
+      p.phase = 2;
+      if (k % 2 == 0) {  // H0 done: H1 starts from its estimates (OmegaModel.ml:202-206)
+        OmegaPair& h1 = st[k + 1];
+        std::copy(p.qs, p.qs + pcsf::kOmegaSettings, h1.qs);
+        h1.qs[1] = omega_h1;
+        h1.qs[2] = sigma_h1;
+        h1.rho = p.rho;
+        h1.kappa = p.kappa;
+        h1.waiting = false;
+        h1.mx.start(h1.rho, 0.001, 10.0, 0.01, rel_tol);
+      }
+    }
+    for (size_t k = 0; k + 1 < np; k += 2)  // an H0 that failed takes its H1 along
+      if (st[k].status != PCSF_OK && st[k + 1].waiting) {
+        st[k + 1].status = st[k].status;
+        st[k + 1].waiting = false;
+      }
+  }
+  if (precision != PCSF_FP64) {
+    // each hypothesis's lpr at its estimates in FP64 (the fits ran on the
+    // MFMA precision's likelihoods): the last kappa step's f = lpr_kappa + lpr
+    std::vector<size_t> need;
+    for (size_t k = 0; k < np; ++k) {
+      act[k] = st[k].status == PCSF_OK;
+      rho[k] = st[k].rho;
+      if (act[k] && st[k].kappa != st[k].eig_kappa) need.push_back(k);
+    }
+    for (size_t k : need)
+      if (!build(k, st[k].kappa)) {
+        st[k].status = PCSF_EFAIL;
+        act[k] = 0;
+      }
+    rc = forward_regions(b, rho.data(), act.data(), PCSF_FP64, &sm);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < np; ++k)
+      if (act[k]) {
+        st[k].lpr = pcsf::omega_lpr_kappa(st[k].kappa) + L[k];
+        st[k].evals++;
+      }
+  }
+  return PCSF_OK;
+}
+
 }  // namespace
 
 int pcsf_score(pcsf_batch* b, int strategy, int precision, double* score, double* anc_comp,
@@ -1482,6 +1745,7 @@ int pcsf_score(pcsf_batch* b, int strategy, int precision, double* score, double
   if (!b) return PCSF_EINVAL;
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
+  if (m->tree_only) return set_err(c, PCSF_EINVAL, kTreeOnly);
   if (m->nm != 2)
     return set_err(c, PCSF_EINVAL, "pcsf_score: needs exactly two models (coding, noncoding)");
   const int R = b->nreg;
@@ -1525,11 +1789,47 @@ int pcsf_score(pcsf_batch* b, int strategy, int precision, double* score, double
   return PCSF_OK;
 }
 
+int pcsf_score_omega(pcsf_batch* b, double omega_h1, double sigma_h1, int precision,
+                     double* score, double* lpr, double* rho, double* kappa, int32_t* status,
+                     int32_t* n_evals) {
+  if (!b) return PCSF_EINVAL;
+  pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  if (m->nm != 2)
+    return set_err(c, PCSF_EINVAL, "pcsf_score_omega: needs a model with two hypothesis slots");
+  if (precision < PCSF_FP64 || precision > PCSF_BF16)
+    return set_err(c, PCSF_EINVAL, "pcsf_score_omega: unknown precision");
+  if (!(omega_h1 >= 0.0) || !(sigma_h1 >= 0.0) || !std::isfinite(omega_h1) ||
+      !std::isfinite(sigma_h1))
+    return set_err(c, PCSF_EINVAL,
+                   "CamlPaml.P14n.instantiate_q: domain violation on variable 1");
+  HIP_TRY(c, hipSetDevice(c->device));
+  std::vector<OmegaPair> st;
+  int rc = score_omega(b, precision, omega_h1, sigma_h1, st);
+  if (rc) return rc;
+  const double db = 10.0 / std::log(10.0);
+  for (int r = 0; r < b->nreg; ++r) {
+    const OmegaPair &h0 = st[(size_t)r * 2], &h1 = st[(size_t)r * 2 + 1];
+    const int s = h0.status ? h0.status : h1.status;
+    if (score) score[r] = s ? NAN : db * (h1.lpr - h0.lpr);
+    for (int h = 0; h < 2; ++h) {
+      const OmegaPair& p = st[(size_t)r * 2 + h];
+      if (lpr) lpr[r * 2 + h] = s ? NAN : p.lpr;
+      if (rho) rho[r * 2 + h] = p.rho;
+      if (kappa) kappa[r * 2 + h] = p.kappa;
+    }
+    if (status) status[r] = s;
+    if (n_evals) n_evals[r] = h0.evals + h1.evals;
+  }
+  return PCSF_OK;
+}
+
 int pcsf_backward(pcsf_batch* b, const double* grad_loglik, double* grad_rho,
                   double* grad_branch) {
   if (!b || !grad_loglik) return PCSF_EINVAL;
   pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
+  if (m->tree_only) return set_err(c, PCSF_EINVAL, kTreeOnly);
   const size_t np = (size_t)b->nreg * m->nm;
   if (b->last_rho.size() != np)
     return set_err(c, PCSF_EINVAL, "pcsf_backward: no forward on this batch to differentiate");
@@ -1580,15 +1880,23 @@ int pcsf_batch_last_timing(const pcsf_batch* b, double* prune_ms, double* reduce
   return PCSF_OK;
 }
 
-int pcsf_lik_columns(pcsf_ctx* c, int k, int n, const int32_t* children, const double* P,
-                     const double* prior, const int32_t* leaves, int64_t ncols, double* z,
-                     double* root_post) {
+}  // extern "C"
+
+namespace {
+
+// PhyloLik.prepare + likelihood + node_posterior + add_branch_posteriors
+// (lib/CamlPaml/PhyloLik.ml:46-174) over n_cols columns: the host side of
+// kernels/lik.hip.  Columns go in chunks whose device buffers stay within
+// ~512 MB; post_mode 1 = the root's posterior, 2 = every node's.
+int lik_impl(pcsf_ctx* c, int k, int n, const int32_t* children, const double* P,
+             const double* prior, const int32_t* leaves, const double* leaf_dist, int64_t ncols,
+             const double* weights, double* z, int post_mode, double* post, double* ecounts) {
   if (!c) return PCSF_EINVAL;
   if (k < 1 || k > 64 || n < 3 || n % 2 == 0 || !children || !P || !prior || !z ||
       (ncols > 0 && !leaves) || ncols < 0)
     return set_err(c, PCSF_EINVAL, "CamlPaml.Infer.prepare: bad arguments");
   const int nl = (n + 1) / 2;
-  std::vector<int32_t> lc(n, -1), rc(n, -1), par(n, -1);
+  std::vector<int32_t> lc(n, -1), rc(n, -1), par(n, -1), sib(n, -1);
   for (int i = nl; i < n; ++i) {
     const int a = children[(i - nl) * 2], b = children[(i - nl) * 2 + 1];
     if (a < 0 || b < 0 || a >= i || b >= i || par[a] >= 0 || par[b] >= 0 || a == b)
@@ -1596,50 +1904,135 @@ int pcsf_lik_columns(pcsf_ctx* c, int k, int n, const int32_t* children, const d
     lc[i] = a;
     rc[i] = b;
     par[a] = par[b] = i;
+    sib[a] = b;
+    sib[b] = a;
   }
-  if ((size_t)n * k * 8 > 65536)
-    return set_err(c, PCSF_EINVAL, "pcsf_lik_columns: workspace exceeds 64 KiB of LDS");
+  bool dist = false;
+  for (int64_t e = 0; e < ncols * nl; ++e) {
+    // a state >= k is a leaf vector of zeros, as raw_leaf makes it
+    // (PhyloLik.ml:11); below -2 is no leaf kind
+    const int v = leaves[e];
+    if (v < -2) return set_err(c, PCSF_EINVAL, "CamlPaml.Infer.prepare: leaf state out of range");
+    dist = dist || v == -2;
+  }
+  if (dist && !leaf_dist)
+    return set_err(c, PCSF_EINVAL, "CamlPaml.Infer.prepare: `Distribution leaves need leaf_dist");
   if (ncols == 0) return PCSF_OK;
   HIP_TRY(c, hipSetDevice(c->device));
-  int32_t *d_lc = nullptr, *d_rc = nullptr, *d_leaves = nullptr;
-  double *d_P = nullptr, *d_prior = nullptr, *d_z = nullptr, *d_post = nullptr;
-  auto cleanup = [&]() {
-    dfree(d_lc);
-    dfree(d_rc);
-    dfree(d_leaves);
-    dfree(d_P);
-    dfree(d_prior);
-    dfree(d_z);
-    dfree(d_post);
-  };
+  const bool estep = ecounts != nullptr;
+  const size_t per_col = (size_t)nl * 4 + (dist ? (size_t)nl * k * 8 : 0) + 16 +
+                         (post_mode == 2 ? (size_t)n * k * 8 : post_mode == 1 ? (size_t)k * 8 : 0);
+  const int64_t chunk =
+      std::max<int64_t>(1, std::min<int64_t>(estep ? 512 : 65536, ((size_t)512 << 20) / per_col));
+  const int grid_max = (int)std::min<int64_t>(estep ? chunk : 2048, std::min(ncols, chunk));
+  int32_t *d_lc = nullptr, *d_rc = nullptr, *d_par = nullptr, *d_sib = nullptr,
+          *d_leaves = nullptr;
+  double *d_P = nullptr, *d_prior = nullptr, *d_dist = nullptr, *d_w = nullptr, *d_z = nullptr,
+         *d_post = nullptr, *d_ws = nullptr, *d_U = nullptr, *d_V = nullptr, *d_ec = nullptr;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) {
     if (e == hipSuccess) e = x;
   };
+  const int64_t cmax = std::min(ncols, chunk);
   chk(dalloc(&d_lc, (size_t)n));
   chk(dalloc(&d_rc, (size_t)n));
-  chk(dalloc(&d_leaves, (size_t)ncols * nl));
+  chk(dalloc(&d_par, (size_t)n));
+  chk(dalloc(&d_sib, (size_t)n));
   chk(dalloc(&d_P, (size_t)(n - 1) * k * k));
   chk(dalloc(&d_prior, (size_t)k));
-  chk(dalloc(&d_z, (size_t)ncols));
-  if (root_post) chk(dalloc(&d_post, (size_t)ncols * k));
-  if (e == hipSuccess) {
-    chk(hipMemcpy(d_lc, lc.data(), n * 4, hipMemcpyHostToDevice));
-    chk(hipMemcpy(d_rc, rc.data(), n * 4, hipMemcpyHostToDevice));
-    chk(hipMemcpy(d_leaves, leaves, (size_t)ncols * nl * 4, hipMemcpyHostToDevice));
-    chk(hipMemcpy(d_P, P, (size_t)(n - 1) * k * k * 8, hipMemcpyHostToDevice));
-    chk(hipMemcpy(d_prior, prior, (size_t)k * 8, hipMemcpyHostToDevice));
+  chk(dalloc(&d_leaves, (size_t)cmax * nl));
+  if (dist) chk(dalloc(&d_dist, (size_t)cmax * nl * k));
+  if (weights) chk(dalloc(&d_w, (size_t)cmax));
+  chk(dalloc(&d_z, (size_t)cmax));
+  if (post_mode) chk(dalloc(&d_post, (size_t)cmax * (post_mode == 2 ? n : 1) * k));
+  chk(dalloc(&d_ws, (size_t)grid_max * 2 * n * k));
+  if (estep) {
+    chk(dalloc(&d_U, (size_t)(n - 1) * grid_max * k));
+    chk(dalloc(&d_V, (size_t)(n - 1) * grid_max * k));
+    chk(dalloc(&d_ec, (size_t)(n - 1) * k * k));
   }
-  if (e == hipSuccess)
-    chk(launch_lik_columns(k, n, d_lc, d_rc, d_P, d_prior, d_leaves, ncols, d_z, d_post,
-                           c->stream));
-  if (e == hipSuccess) chk(hipStreamSynchronize(c->stream));
-  if (e == hipSuccess) chk(hipMemcpy(z, d_z, (size_t)ncols * 8, hipMemcpyDeviceToHost));
-  if (e == hipSuccess && root_post)
-    chk(hipMemcpy(root_post, d_post, (size_t)ncols * k * 8, hipMemcpyDeviceToHost));
-  cleanup();
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  auto down = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  };
+  up(d_lc, lc.data(), n * 4);
+  up(d_rc, rc.data(), n * 4);
+  up(d_par, par.data(), n * 4);
+  up(d_sib, sib.data(), n * 4);
+  up(d_P, P, (size_t)(n - 1) * k * k * 8);
+  up(d_prior, prior, (size_t)k * 8);
+  if (estep) up(d_ec, ecounts, (size_t)(n - 1) * k * k * 8);  // add_branch_posteriors adds
+  for (int64_t c0 = 0; c0 < ncols && e == hipSuccess; c0 += chunk) {
+    const int64_t nc = std::min(chunk, ncols - c0);
+    up(d_leaves, leaves + c0 * nl, (size_t)nc * nl * 4);
+    if (dist) up(d_dist, leaf_dist + (size_t)c0 * nl * k, (size_t)nc * nl * k * 8);
+    if (weights) up(d_w, weights + c0, (size_t)nc * 8);
+    LikArgs a;
+    a.k = k;
+    a.n = n;
+    a.lc = d_lc;
+    a.rc = d_rc;
+    a.parent = d_par;
+    a.sib = d_sib;
+    a.P = d_P;
+    a.prior = d_prior;
+    a.leaves = d_leaves;
+    a.leaf_dist = d_dist;
+    a.ncols = nc;
+    a.weights = d_w;
+    a.ws = d_ws;
+    a.z = d_z;
+    a.post_mode = post_mode;
+    a.post = d_post;
+    a.U = d_U;
+    a.V = d_V;
+    a.ecounts = d_ec;
+    // one column per workgroup when the E-step factors are kept per slot
+    const int grid = estep ? (int)nc : (int)std::min<int64_t>(nc, grid_max);
+    if (e == hipSuccess) e = launch_lik_posteriors(a, grid, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    down(z + c0, d_z, (size_t)nc * 8);
+    if (post_mode)
+      down(post + (size_t)c0 * (post_mode == 2 ? n : 1) * k, d_post,
+           (size_t)nc * (post_mode == 2 ? n : 1) * k * 8);
+  }
+  if (estep) down(ecounts, d_ec, (size_t)(n - 1) * k * k * 8);
+  for (auto* p : {d_lc, d_rc, d_par, d_sib, d_leaves}) {
+    int32_t* q = p;
+    dfree(q);
+  }
+  for (auto* p : {d_P, d_prior, d_dist, d_w, d_z, d_post, d_ws, d_U, d_V, d_ec}) {
+    double* q = p;
+    dfree(q);
+  }
   HIP_TRY(c, e);
   return PCSF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcsf_lik_columns(pcsf_ctx* c, int k, int n, const int32_t* children, const double* P,
+                     const double* prior, const int32_t* leaves, int64_t ncols, double* z,
+                     double* root_post) {
+  if (!c) return PCSF_EINVAL;
+  if (leaves)  // this entry takes `Certain and `Marginalize leaves only
+    for (int64_t e = 0; e < ncols * ((n + 1) / 2); ++e)
+      if (leaves[e] < -1)
+        return set_err(c, PCSF_EINVAL, "CamlPaml.Infer.prepare: leaf state out of range");
+  return lik_impl(c, k, n, children, P, prior, leaves, nullptr, ncols, nullptr, z,
+                  root_post ? 1 : 0, root_post, nullptr);
+}
+
+int pcsf_lik_posteriors(pcsf_ctx* c, int k, int n, const int32_t* children, const double* P,
+                        const double* prior, const int32_t* leaves, const double* leaf_dist,
+                        int64_t ncols, const double* weights, double* z, double* node_post,
+                        double* ecounts) {
+  return lik_impl(c, k, n, children, P, prior, leaves, leaf_dist, ncols, weights, z,
+                  node_post ? 2 : 0, node_post, ecounts);
 }
 
 }  // extern "C"

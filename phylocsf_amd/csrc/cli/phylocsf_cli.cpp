@@ -17,7 +17,9 @@
 //                                 with it to 1e-3 dB per region at fixed scales)
 //   --device=N                    GPU index (default 0)
 //   --batchBytes=N                leaf-code bytes packed per GPU call (default 1 GiB)
-// The omega strategy is not part of this build.
+// The omega strategy (src/OmegaModel.ml) runs every candidate region's two
+// MAP fits in lock step on the GPU (pcsf_score_omega), with omega_H1 and
+// sigma_H1 from the parameter set's optional _omega file (:452-463).
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -78,7 +80,7 @@ struct OcamlError {
 
 // ---- options (src/PhyloCSF.ml:17-50) ---------------------------------------
 
-enum class Strategy { MaxLik, FixedLik, Nop };
+enum class Strategy { MaxLik, FixedLik, Omega, Nop };
 enum class OrfMode { AsIs, ATGStop, StopStop, StopStop3, ToFirstStop, FromLastStop, ToOrFromStop };
 
 struct Options {
@@ -207,10 +209,8 @@ Options parse_options(int argc, char** argv) {
       if (v == "mle") o.strategy = Strategy::MaxLik;
       else if (v == "fixed") o.strategy = Strategy::FixedLik;
       else if (v == "nop") o.strategy = Strategy::Nop;
-      else if (v == "omega") {
-        std::fprintf(stderr, "PhyloCSF: the omega strategy is not part of this build\n");
-        std::exit(2);
-      } else usage_error("option --strategy: invalid strategy " + val);
+      else if (v == "omega") o.strategy = Strategy::Omega;
+      else usage_error("option --strategy: invalid strategy " + val);
     } else if (name == "species") {
       o.species = val;
       o.species_set = true;
@@ -582,6 +582,7 @@ struct Scorer {
   pcsf_model* model = nullptr;
   std::vector<int32_t> children;
   std::vector<double> branch, S, pi;
+  double omega_h1 = 0.2, sigma_h1 = 0.01;  // OmegaModel.score's defaults
 
   explicit Scorer(const Options& opt) : o(opt) {}
   ~Scorer() {
@@ -598,11 +599,18 @@ struct Scorer {
     int rc = pcsf_ctx_create(o.device, &ctx);
     if (rc) device_fail("GPU context", rc);
     const int n = (int)branch.size();
-    rc = pcsf_model_create(ctx, n, children.data(), branch.data(), 2, S.data(), pi.data(), &model);
+    if (o.strategy == Strategy::Omega)  // a tree only: the omega model is made per region
+      rc = pcsf_model_create(ctx, n, children.data(), branch.data(), 0, nullptr, nullptr, &model);
+    else
+      rc = pcsf_model_create(ctx, n, children.data(), branch.data(), 2, S.data(), pi.data(),
+                             &model);
     if (rc) device_fail("model", rc);
   }
   struct Result {
+    // fixed / mle: rho and lpr of the coding and noncoding models; omega: of
+    // H0 and H1, with kappa
     double score = 0, anc = 0, rho_c = 1, rho_n = 1, lpr_c = 0, lpr_n = 0;
+    double kappa_0 = 0, kappa_1 = 0;
     int status = PCSF_OK;
   };
   // scores regions [codes rows per region given by offsets]
@@ -615,16 +623,21 @@ struct Scorer {
     pcsf_batch* b = nullptr;
     int rc = pcsf_batch_create(model, codes.data(), off.back(), off.data(), R, &b);
     if (rc) device_fail("batch", rc);
-    std::vector<double> sc(R), an(R), rho(2 * R), lpr(2 * R);
+    std::vector<double> sc(R), an(R, NAN), rho(2 * R), lpr(2 * R), kap(2 * R, 0.0);
     std::vector<int32_t> st(R);
-    rc = pcsf_score(b, o.strategy == Strategy::MaxLik ? PCSF_STRATEGY_MLE : PCSF_STRATEGY_FIXED,
-                    o.precision, sc.data(), an.data(), rho.data(), lpr.data(), nullptr,
-                    st.data(), nullptr);
+    if (o.strategy == Strategy::Omega)
+      rc = pcsf_score_omega(b, omega_h1, sigma_h1, o.precision, sc.data(), lpr.data(), rho.data(),
+                            kap.data(), st.data(), nullptr);
+    else
+      rc = pcsf_score(b, o.strategy == Strategy::MaxLik ? PCSF_STRATEGY_MLE : PCSF_STRATEGY_FIXED,
+                      o.precision, sc.data(), an.data(), rho.data(), lpr.data(), nullptr,
+                      st.data(), nullptr);
     if (rc) device_fail("scoring", rc);
     *err = pcsf_last_error(ctx);
     pcsf_batch_destroy(b);
     for (int32_t r = 0; r < R; ++r)
-      res[r] = Result{sc[r], an[r], rho[2 * r], rho[2 * r + 1], lpr[2 * r], lpr[2 * r + 1], st[r]};
+      res[r] = Result{sc[r],         an[r],       rho[2 * r],  rho[2 * r + 1], lpr[2 * r],
+                      lpr[2 * r + 1], kap[2 * r], kap[2 * r + 1], st[r]};
     return res;
   }
 };
@@ -663,7 +676,40 @@ class Driver {
     }
     t_ = pcsf::tree_of_newick(*nt_);
     for (int i = 0; i < t_.nl; ++i) t_species_.insert(t_.labels[i]);
-    if (o.strategy != Strategy::Nop) {
+    if (o.strategy == Strategy::Omega) {
+      // initialize_strategy's OmegaTest (src/PhyloCSF.ml:452-463): an optional
+      // key<TAB>value file naming omega_H1 and sigma_H1
+      const std::string fn = paramfile(paramset, "_omega", false);
+      if (file_exists(fn)) {
+        std::map<std::string, std::string> kv;
+        for (std::string line : split_lines(read_file(fn))) {
+          const size_t a = line.find_first_not_of(" \t"), b = line.find_last_not_of(" \t");
+          line = (a == std::string::npos) ? "" : line.substr(a, b - a + 1);
+          if (line.empty() || line[0] == '#') continue;
+          const size_t tab = line.find('\t');
+          if (tab == std::string::npos) kv[line] = "";
+          else kv[line.substr(0, tab)] = line.substr(tab + 1);
+        }
+        auto num = [&](const char* key, double* out) {
+          auto it = kv.find(key);
+          if (it == kv.end()) return false;
+          char* end = nullptr;
+          *out = std::strtod(it->second.c_str(), &end);
+          return end && end != it->second.c_str() && *end == '\0';
+        };
+        if (!num("omega_H1", &scorer_.omega_h1) || !num("sigma_H1", &scorer_.sigma_h1))
+          failwith("configuration file " + fn +
+                   " exists but does not specify valid omega_H1 and sigma_H1 values");
+      }
+    }
+    if (o.strategy == Strategy::Omega) {
+      scorer_.nl = t_.nl;
+      for (int i = t_.nl; i < t_.n; ++i) {
+        scorer_.children.push_back(t_.lc[i]);
+        scorer_.children.push_back(t_.rc[i]);
+      }
+      scorer_.branch = t_.branch;
+    } else if (o.strategy != Strategy::Nop) {
       const std::string fc = paramfile(paramset, "_coding.ECM", true);
       const std::string fnc = paramfile(paramset, "_noncoding.ECM", true);
       scorer_.S.resize(2 * 4096);
@@ -784,6 +830,12 @@ class Driver {
               {"L(C)", dbs(r.lpr_c)}, {"L(NC)", dbs(r.lpr_n)}};
     if (o_.strategy == Strategy::FixedLik)
       return {{"rho", sf(1.0)}, {"L(C)", dbs(r.lpr_c)}, {"L(NC)", dbs(r.lpr_n)}};
+    if (o_.strategy == Strategy::Omega)  // src/OmegaModel.ml:210-215
+      return {{"L(H0)", dbs(r.lpr_c)},         {"rho_H0", sf(r.rho_c)},
+              {"kappa_H0", sf(r.kappa_0)},     {"omega_H0", sf(1.0)},
+              {"sigma_H0", sf(1.0)},           {"L(H1)", dbs(r.lpr_n)},
+              {"rho_H1", sf(r.rho_n)},         {"kappa_H1", sf(r.kappa_1)},
+              {"omega_H1", sf(scorer_.omega_h1)}, {"sigma_H1", sf(scorer_.sigma_h1)}};
     return {};
   }
 
@@ -828,7 +880,9 @@ class Driver {
     // List.reduce max: OCaml's structural order on (score record, rc, lo, hi)
     auto less = [](const Scored& a, const Scored& b) {
       if (a.r.score != b.r.score) return a.r.score < b.r.score;
-      if (a.r.anc != b.r.anc) return a.r.anc < b.r.anc;
+      // OCaml's compare: nan equals nan (the omega test's anc_comp_score)
+      if (!(std::isnan(a.r.anc) && std::isnan(b.r.anc)) && a.r.anc != b.r.anc)
+        return a.r.anc < b.r.anc;
       if (a.diag != b.diag) return a.diag < b.diag;
       if (a.rg.rc != b.rg.rc) return !a.rg.rc;
       if (a.rg.lo != b.rg.lo) return a.rg.lo < b.rg.lo;

@@ -186,4 +186,140 @@ void BrentMin::accept(double fu) {
   }
 }
 
+namespace {
+const std::vector<double>& retries_of(double lo, double hi) {
+  // the retry abscissae depend only on (lo, hi): Random.init 0 each time
+  static thread_local double c_lo = -1, c_hi = -1;
+  static thread_local std::vector<double> c;
+  if (lo != c_lo || hi != c_hi) {
+    c = find_init_retries(lo, hi, 250);
+    c_lo = lo;
+    c_hi = hi;
+  }
+  return c;
+}
+}  // namespace
+
+void MaximizeLpr::start(double init, double lo, double hi, double accuracy, double rel_tol) {
+  lo_ = lo;
+  hi_ = hi;
+  acc_ = accuracy;
+  rel_tol_ = rel_tol;
+  xc_ = init;
+  tries_ = 0;
+  failed = false;
+  evals = 0;
+  brent_started_ = false;
+  seen_x_.clear();
+  seen_f_.clear();
+  phase_ = kLo;
+  pending_ = lo;
+}
+
+bool MaximizeLpr::recall(double xx, double* f) const {
+  for (size_t k = seen_x_.size(); k-- > 0;)
+    if (seen_x_[k] == xx) {
+      *f = seen_f_[k];
+      return true;
+    }
+  return false;
+}
+
+// Fit.find_init's loop condition and exit (lib/CamlPaml/Fit.ml:36-48), then
+// the start of the maximiser when the point is interior.
+void MaximizeLpr::after_find_init() {
+  if (tries_ < 250 && (fc_ <= flo_ || fc_ <= fhi_)) {
+    xc_ = retries_of(lo_, hi_)[tries_];
+    ++tries_;
+    phase_ = kRetry;
+    pending_ = xc_;
+    return;
+  }
+  // Fit.ml:43-47: after the 250th try the better end is taken, whatever that
+  // try gave
+  if (tries_ == 250) xc_ = (flo_ > fhi_) ? lo_ : hi_;
+  if (lo_ < xc_ && xc_ < hi_) {
+    // gsl_min_fminimizer_set: lower, upper and the minimum must be finite
+    // and enclose the minimum of -f
+    const double fmin = -fc_, flo = -flo_, fhi = -fhi_;
+    if (!std::isfinite(fmin) || !std::isfinite(flo) || !std::isfinite(fhi) ||
+        !(fmin < flo && fmin < fhi)) {
+      failed = true;
+      phase_ = kDone;
+      return;
+    }
+    // brent_init's f at the golden point
+    br_.x_lo = lo_;
+    br_.x_hi = hi_;
+    phase_ = kBrent;
+    brent_started_ = false;
+    pending_ = br_.golden_point();
+    return;
+  }
+  phase_ = kFinal;
+  pending_ = xc_;
+}
+
+void MaximizeLpr::brent_next() {
+  br_.propose();
+  pending_ = br_.u;
+}
+
+void MaximizeLpr::feed(double f) {
+  ++evals;
+  seen_x_.push_back(pending_);
+  seen_f_.push_back(f);
+  take(f);
+  double known;
+  while (phase_ != kDone && recall(pending_, &known)) take(known);
+}
+
+void MaximizeLpr::take(double f) {
+  switch (phase_) {
+    case kLo:
+      flo_ = f;
+      phase_ = kHi;
+      pending_ = hi_;
+      break;
+    case kHi:
+      fhi_ = f;
+      phase_ = kInit;
+      pending_ = xc_;
+      break;
+    case kInit:
+    case kRetry:
+      fc_ = f;
+      after_find_init();
+      break;
+    case kBrent:
+      if (!brent_started_) {
+        br_.init(xc_, -fc_, lo_, -flo_, hi_, -fhi_, -f);
+        br_.rel_tol = rel_tol_;
+        brent_started_ = true;
+        brent_next();
+        break;
+      }
+      if (!std::isfinite(f)) {
+        failed = true;
+        phase_ = kDone;
+        break;
+      }
+      br_.accept(-f);
+      if ((br_.x_hi - br_.x_lo) / br_.x_min > acc_) {
+        brent_next();
+      } else {
+        phase_ = kFinal;
+        pending_ = br_.x_min;
+      }
+      break;
+    case kFinal:
+      x = pending_;
+      fx = f;
+      phase_ = kDone;
+      return;
+    default:
+      return;
+  }
+}
+
 }  // namespace pcsf

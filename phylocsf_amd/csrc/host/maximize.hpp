@@ -49,4 +49,37 @@ struct BrentMin {
 
 constexpr double kGolden = 0.3819660;  // GSL's (3 - sqrt 5) / 2
 
+// PhyloCSFModel.maximize_lpr ~init ~lo ~hi ~accuracy f g
+// (src/PhyloCSFModel.ml:84-99) for one function, one evaluation at a time:
+// Fit.find_init (f at lo, hi, init, then Random.init 0 retries in log space,
+// at most 250) and, when its point is interior, GSL Brent until the bracket
+// is within `accuracy` of the estimate; the result is (x, f x).  Values at
+// abscissae already evaluated are reused (the reference evaluates them again
+// and gets the same numbers), so pending() is always a new abscissa.
+class MaximizeLpr {
+ public:
+  void start(double init, double lo, double hi, double accuracy, double rel_tol);
+  bool done() const { return phase_ == kDone; }
+  double pending() const { return pending_; }  // the abscissa to evaluate next
+  void feed(double f);                         // f(pending())
+  double x = 0, fx = 0;  // result once done
+  bool failed = false;   // a non-finite value where Brent needs a finite one
+  int evals = 0;         // evaluations fed
+
+ private:
+  enum Phase { kLo, kHi, kInit, kRetry, kBrent, kFinal, kDone };
+  void after_find_init();
+  void brent_next();
+  bool recall(double xx, double* f) const;
+  void take(double f);  // one step of the state machine with f(pending())
+  Phase phase_ = kDone;
+  double lo_ = 0, hi_ = 0, acc_ = 0, rel_tol_ = 0;
+  double flo_ = 0, fhi_ = 0, xc_ = 0, fc_ = 0;
+  int tries_ = 0;
+  BrentMin br_;
+  bool brent_started_ = false;
+  double pending_ = 0;
+  std::vector<double> seen_x_, seen_f_;
+};
+
 }  // namespace pcsf

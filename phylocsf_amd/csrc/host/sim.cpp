@@ -93,7 +93,8 @@ int draw(const double* cum, int k, double u) {
 static int simulate(pcsf_model* m, int32_t nreg, const int64_t* off, const int32_t* region_model,
              const double* region_rho, int gap_mode, double gap_p, double gap_run, double n_p,
              uint64_t seed, int n_threads, uint8_t* codes, uint8_t* nuc) {
-  if (!m || nreg < 0 || !off || !region_model || !region_rho || (!codes && !nuc))
+  if (!m || m->tree_only || nreg < 0 || !off || !region_model || !region_rho ||
+      (!codes && !nuc))
     return PCSF_EINVAL;
   if (gap_p < 0 || gap_p >= 1 || n_p < 0 || n_p > 1 || (gap_mode == 1 && !(gap_run >= 1)))
     return PCSF_EINVAL;

@@ -51,6 +51,14 @@ struct PSets {
   // cherry c above its two leaves a, b, normalised to max 1, and its ln max
   float* d_ctab = nullptr;     // cap * nctab * 65 * 65 * 64
   double* d_clnt = nullptr;    // cap * nctab * 65 * 65
+  // rate models per set (allocated on request: a model whose Q differs from
+  // set to set, as the omega test's per region and hypothesis); nullptr: set
+  // s uses model s % nm's eigensystem and root prior
+  double* d_eVT = nullptr;     // cap * 4096, V transposed
+  double* d_eVinv = nullptr;   // cap * 4096
+  double* d_elam = nullptr;    // cap * 64
+  double* d_epi = nullptr;     // cap * 64 root prior
+  double* d_elnpi = nullptr;   // cap * 64 its logarithm
 };
 
 struct pcsf_ctx {
@@ -64,6 +72,9 @@ struct pcsf_ctx {
 struct pcsf_model {
   pcsf_ctx* ctx = nullptr;
   int n = 0, nl = 0, nm = 0, nbr = 0;
+  // a tree without rate models (made with n_models = 0, for the omega test,
+  // whose sets carry rate models of their own): nm = 2 hypotheses
+  bool tree_only = false;
   std::vector<int> lc, rc, parent;
   std::vector<double> branch;
   std::vector<pcsf::Eigen> eig;
@@ -194,6 +205,23 @@ hipError_t launch_prune_mfma(const pcsf_model* m, const pcsf_batch* b, int preci
 // with or without cherry tables), or nullptr.
 const void* find_tree_program(const std::vector<int32_t>& ops, bool tables);
 hipError_t launch_region_reduce(const pcsf_batch* b, hipStream_t s);
-hipError_t launch_lik_columns(int k, int n, const int32_t* d_lc, const int32_t* d_rc,
-                              const double* d_P, const double* d_prior, const int32_t* d_leaves,
-                              int64_t ncols, double* d_z, double* d_post, hipStream_t s);
+// The PhyloLik compatibility path (kernels/lik.hip), device pointers.
+struct LikArgs {
+  int k = 0, n = 0;
+  const int32_t *lc = nullptr, *rc = nullptr, *parent = nullptr, *sib = nullptr;
+  const double* P = nullptr;      // (n-1) x k x k, row-major per branch
+  const double* prior = nullptr;  // k
+  const int32_t* leaves = nullptr;  // ncols x nl: state, -1 `Marginalize, -2 `Distribution
+  const double* leaf_dist = nullptr;  // ncols x nl x k (rows used where leaves = -2)
+  int64_t ncols = 0;
+  const double* weights = nullptr;  // ncols (null: 1)
+  double* ws = nullptr;             // grid x 2n x k workspace
+  double* z = nullptr;              // ncols
+  int post_mode = 0;                // 0 none, 1 root (ncols x k), 2 every node (ncols x n x k)
+  double* post = nullptr;
+  double* U = nullptr;              // (n-1) x grid x k, with V: the E-step factors
+  double* V = nullptr;
+  double* ecounts = nullptr;        // (n-1) x k x k, added to
+};
+// grid workgroups; with U set, ncols must not exceed grid
+hipError_t launch_lik_posteriors(const LikArgs& a, int grid, hipStream_t s);

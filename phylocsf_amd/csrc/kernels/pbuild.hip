@@ -382,8 +382,13 @@ hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int 
   const int nbw = ((size_t)nsets * m->nbr >= 16384) ? 8 : 1;
   dim3 grid((m->nbr + nbw - 1) / nbw, nsets);
   auto* k = sets.d_Dfrag ? pbuild_kernel<true> : pbuild_kernel<false>;
-  hipLaunchKernelGGL(k, grid, dim3(256), 0, s, m->d_VT, m->d_Vinv, m->d_lam,
-                     m->d_branch, sets.d_rho, sets.d_active, m->nm, m->nbr, nbw, m->nl,
+  // the eigensystem of set s is model s % nm's, or the set's own when the
+  // sets carry rate models (then s % cap = s)
+  const bool own = sets.d_eVT != nullptr;
+  hipLaunchKernelGGL(k, grid, dim3(256), 0, s, own ? sets.d_eVT : m->d_VT,
+                     own ? sets.d_eVinv : m->d_Vinv, own ? sets.d_elam : m->d_lam,
+                     m->d_branch, sets.d_rho, sets.d_active, own ? sets.cap : m->nm, m->nbr,
+                     nbw, m->nl,
                      sets.fp64 ? sets.d_P64T : nullptr, sets.d_PT32, sets.d_Pfrag, fmt,
                      sets.d_lnPcc, m->d_perr, sets.d_DT32,
                      sets.d_Dfrag);

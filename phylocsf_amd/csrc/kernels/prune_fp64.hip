@@ -72,8 +72,8 @@ __global__ __launch_bounds__(64) void prune_fp64_regions_kernel(
     const double* __restrict__ P64T, const double* __restrict__ pi,
     const double* __restrict__ lnpi, const int32_t* __restrict__ block_region,
     const int64_t* __restrict__ astart, const int64_t* __restrict__ off,
-    const uint8_t* __restrict__ active, int r0, double* __restrict__ logz,
-    double* __restrict__ colelpr) {
+    const uint8_t* __restrict__ active, int r0, int prior_per_set,
+    double* __restrict__ logz, double* __restrict__ colelpr) {
   extern __shared__ double alpha[];  // (n - nl) x 64
   const int a = threadIdx.x;
   const int64_t ntask = ncols * nm;
@@ -107,8 +107,9 @@ __global__ __launch_bounds__(64) void prune_fp64_regions_kernel(
     }
     if (a == 0) {
       const double* ar = alpha + (size_t)(n - 1 - nl) * 64;
-      const double* p = pi + q * 64;
-      const double* pl = lnpi + q * 64;
+      const int pq = prior_per_set ? set : q;  // one prior per set, or per model
+      const double* p = pi + pq * 64;
+      const double* pl = lnpi + pq * 64;
       double z = 0.0;
       for (int x = 0; x < 64; ++x) z += ar[x] * p[x];
       double e = 0.0;
@@ -117,45 +118,6 @@ __global__ __launch_bounds__(64) void prune_fp64_regions_kernel(
       logz[(size_t)q * ld + col] = log(z);
       colelpr[(size_t)q * ld + col] = e;
     }
-    __syncthreads();
-  }
-}
-
-// Generic alphabet (k <= 64) PhyloLik.prepare/likelihood/node_posterior root
-// for caller-supplied row-major P (the CamlPaml compatibility entry).
-__global__ __launch_bounds__(64) void lik_columns_kernel(
-    int k, int n, const int32_t* __restrict__ lc, const int32_t* __restrict__ rc,
-    const double* __restrict__ P, const double* __restrict__ prior,
-    const int32_t* __restrict__ leaves, int64_t ncols, double* __restrict__ zout,
-    double* __restrict__ post) {
-  extern __shared__ double alpha[];  // n x k (leaf vectors too)
-  const int a = threadIdx.x;
-  const int nl = (n + 1) / 2;
-  for (int64_t col = blockIdx.x; col < ncols; col += gridDim.x) {
-    if (a < k)
-      for (int j = 0; j < nl; ++j) {
-        const int c = leaves[col * nl + j];
-        alpha[j * k + a] = (c < 0) ? 1.0 : (c == a ? 1.0 : 0.0);
-      }
-    __syncthreads();
-    for (int i = nl; i < n; ++i) {
-      if (a < k) {
-        const double* Pl = P + (size_t)lc[i] * k * k + (size_t)a * k;
-        const double* Pr = P + (size_t)rc[i] * k * k + (size_t)a * k;
-        const double* al = alpha + (size_t)lc[i] * k;
-        const double* ar = alpha + (size_t)rc[i] * k;
-        double x = 0.0, y = 0.0;
-        for (int b = 0; b < k; ++b) x += Pl[b] * al[b];
-        for (int b = 0; b < k; ++b) y += Pr[b] * ar[b];
-        alpha[(size_t)i * k + a] = x * y;
-      }
-      __syncthreads();
-    }
-    const double* aroot = alpha + (size_t)(n - 1) * k;
-    double z = 0.0;
-    for (int x = 0; x < k; ++x) z += aroot[x] * prior[x];
-    if (a == 0) zout[col] = z;
-    if (post && a < k) post[col * k + a] = (z == 0.0) ? 0.0 : aroot[a] * prior[a] / z;
     __syncthreads();
   }
 }
@@ -184,19 +146,9 @@ hipError_t launch_prune_fp64_regions(const pcsf_model* m, const pcsf_batch* b,
   if (grid <= 0) return hipSuccess;
   hipLaunchKernelGGL(prune_fp64_regions_kernel, dim3(grid), dim3(64), lds, s, b->al.d_codes, c0,
                      ncols, b->al.ld, m->n, m->nl, m->nm, m->nbr, m->d_lc, m->d_rc,
-                     ch.sets->d_P64T, m->d_pi, m->d_lnpi, b->al.d_block_region, b->al.d_astart,
-                     b->d_offsets, ch.sets->d_active, ch.r0, b->d_alogz, b->d_acolelpr);
-  return hipGetLastError();
-}
-
-hipError_t launch_lik_columns(int k, int n, const int32_t* d_lc, const int32_t* d_rc,
-                              const double* d_P, const double* d_prior, const int32_t* d_leaves,
-                              int64_t ncols, double* d_z, double* d_post, hipStream_t s) {
-  const size_t lds = (size_t)n * k * sizeof(double);
-  if (lds > 65536) return hipErrorInvalidValue;
-  const int grid = (int)std::min<int64_t>(ncols, 8192);
-  if (grid <= 0) return hipSuccess;
-  hipLaunchKernelGGL(lik_columns_kernel, dim3(grid), dim3(64), lds, s, k, n, d_lc, d_rc, d_P,
-                     d_prior, d_leaves, ncols, d_z, d_post);
+                     ch.sets->d_P64T, ch.sets->d_epi ? ch.sets->d_epi : m->d_pi,
+                     ch.sets->d_epi ? ch.sets->d_elnpi : m->d_lnpi, b->al.d_block_region,
+                     b->al.d_astart, b->d_offsets, ch.sets->d_active, ch.r0,
+                     ch.sets->d_epi ? 1 : 0, b->d_alogz, b->d_acolelpr);
   return hipGetLastError();
 }

@@ -57,8 +57,9 @@ MfmaParams base_params(const pcsf_model* m, const PSets& sets) {
   prm.Pfrag = sets.d_Pfrag;
   prm.PT32 = sets.d_PT32;
   prm.lnPcc = sets.d_lnPcc;
-  prm.pi = m->d_pi;
-  prm.lnpi = m->d_lnpi;
+  prm.pi = sets.d_epi ? sets.d_epi : m->d_pi;
+  prm.lnpi = sets.d_epi ? sets.d_elnpi : m->d_lnpi;
+  prm.prior_per_set = sets.d_epi ? 1 : 0;
   prm.ls_leaves = tables ? m->d_lsl_t : m->d_lsl;
   prm.n_ls = tables ? m->n_lsl_t : m->n_lsl;
   return prm;

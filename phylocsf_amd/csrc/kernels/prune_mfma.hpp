@@ -99,6 +99,9 @@ struct MfmaParams {
   // ln P[c][c] are summed once per task, before the walk
   const int16_t* ls_leaves;
   int n_ls;
+  // pi / lnpi hold one root prior per set instead of per model (sets with
+  // rate models of their own)
+  int prior_per_set;
 };
 
 struct Task {
@@ -557,8 +560,9 @@ struct Walk {
   }
   // ln z and elpr of this lane's columns from the root tiles
   __device__ __forceinline__ void root(const T& s) {
-    const double* piq = p.pi + q * 64;
-    const double* lnq = p.lnpi + q * 64;
+    const int pq = p.prior_per_set ? set : q;
+    const double* piq = p.pi + pq * 64;
+    const double* lnq = p.lnpi + pq * 64;
 #pragma unroll
     for (int c = 0; c < G; ++c) {
       double z = 0.0, el = 0.0;

@@ -84,3 +84,42 @@ def test_batched_files_match_single_runs(tmp_path, examples_dir):
     for f in fas + fas[:1]:
         single += run("29mammals", f, *args)
     assert both == single
+
+
+def test_omega_strategy_against_oracle(examples_dir, params_dir, tmp_path):
+    """--strategy=omega (src/PhyloCSF.ml:452-463, src/OmegaModel.ml:195-219)
+    against the oracle's restatement: scores as printed, the max region, and
+    the diagnostics; H1 from a parameter set's _omega file."""
+    import shutil
+    cases = [("12flies", "tal-AA.fa", []), ("29mammals", "ALDH2.exon5.fa", ["--frames=3"])]
+    for pset, fa, extra in cases:
+        _say("omega strategy, " + fa)
+        out = run(pset, os.path.join(examples_dir, fa), "--strategy=omega", "--debug",
+                  "--precision=fp64", "--ancComp", *extra)[0]
+        frames = 3 if extra else 1
+        res, best = CO.run(os.path.join(params_dir, pset), open(os.path.join(examples_dir, fa)).read(),
+                           strategy="omega", frames=frames)
+        assert abs(float(out[2]) - best[0]["score"]) <= 6e-5, (out, best)
+        if frames > 1:
+            assert out[3:5] == [str(best[2]), str(best[3])]
+        assert out[-2] == "nan"  # anc_comp_score is nan for this test
+        diag = dict(kv.split("=") for kv in out[-1].lstrip("# ").split(" "))
+        assert diag["omega_H1"] == "0.20" and diag["sigma_H1"] == "0.01"
+        assert diag["rho_H0"] == "%.2f" % best[0]["rho_H0"]
+        assert diag["kappa_H1"] == "%.2f" % best[0]["kappa_H1"]
+    # a parameter set with an _omega file
+    for ext in (".nh",):
+        shutil.copy(os.path.join(params_dir, "12flies" + ext), tmp_path / ("x" + ext))
+    (tmp_path / "x_omega").write_text("# H1\nomega_H1\t0.5\nsigma_H1\t0.3\n")
+    prefix = str(tmp_path / "x")
+    out = run(prefix, os.path.join(examples_dir, "tal-AA.fa"), "--strategy=omega",
+              "--precision=fp64", "--debug")[0]
+    _, best = CO.run(prefix, open(os.path.join(examples_dir, "tal-AA.fa")).read(),
+                     strategy="omega")
+    assert abs(float(out[2]) - best[0]["score"]) <= 6e-5
+    assert "omega_H1=0.50" in out[-1] and "sigma_H1=0.30" in out[-1]
+    (tmp_path / "x_omega").write_text("omega_H1\t0.5\n")
+    e = dict(os.environ)
+    p = subprocess.run([EXE, prefix, os.path.join(examples_dir, "tal-AA.fa"), "--strategy=omega"],
+                       capture_output=True, text=True, env=e, timeout=120)
+    assert p.returncode != 0 and "does not specify valid omega_H1 and sigma_H1" in p.stderr

@@ -158,14 +158,23 @@ def test_c3_full_gradient_maximiser_precisions(ctx, c3_full):
     _say("C3: all %d regions, mle_grad at FP64 / FP32 / split FP16" % wl.n_regions)
     g64 = P.score(b, "mle_grad", P.PCSF_FP64)
     assert (g64["status"] == 0).all()
+    lengths = np.diff(wl.offsets)
+    worst = {}
     for prec, name in ((P.PCSF_FP32, "FP32"), (P.PCSF_FP16S, "split FP16")):
         g = P.score(b, "mle_grad", prec)
         assert (g["status"] == 0).all()
         dd = np.abs(g["score"] - g64["score"])
-        _say("  %s mle_grad vs FP64 mle_grad: max |d score| %.3g dB (mean %.3g), "
-             "max |d ln rho| %.3g" % (name, dd.max(), dd.mean(),
-                                      np.max(np.abs(np.log(g["rho"] / g64["rho"])))))
-        assert dd.max() <= TOL_DB, name
+        # the forward's own rounding at the FP64 estimates
+        ll, _ = b.forward(g64["rho"], prec, per_region=True)
+        fw = np.abs(P.DB * ((ll[:, 0] - ll[:, 1]) - (g64["lpr"][:, 0] - g64["lpr"][:, 1])))
+        rel = np.max(np.abs(ll - g64["lpr"]) / np.abs(g64["lpr"]))
+        i = int(np.argmax(dd))
+        _say("  %s mle_grad vs FP64 mle_grad: max |d score| %.3g dB (mean %.3g; region %d, "
+             "%d codons), max |d ln rho| %.3g; forward at the FP64 estimates: max %.3g dB, "
+             "max rel lpr %.3g" % (name, dd.max(), dd.mean(), i, lengths[i],
+                                   np.max(np.abs(np.log(g["rho"] / g64["rho"]))), fw.max(), rel))
+        worst[name] = dd.max()
+    assert max(worst.values()) <= TOL_DB, worst
 
 
 def test_c3_full_reference_maximiser_within_stopping_bound(ctx, c3_full):
@@ -239,7 +248,7 @@ def test_c4_fp32_vs_fp64_every_frame(ctx, mammals120, params_dir):
     codes, coff = b.codes()
     for k in range(18):
         ref = om.score_fixed(codes[coff[k]:coff[k + 1]])
-        assert abs(s64["score"][k] - ref["score"]) <= 1e-8, k
+        assert abs(s64["score"][k] - ref["score"]) <= 1e-9 + 1e-11 * abs(ref["score"]), k
         assert abs(s64["lpr_c"][k] - ref["lpr_c"]) <= 1e-10 * max(1, abs(ref["lpr_c"])), k
 
 

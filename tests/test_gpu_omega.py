@@ -1,0 +1,116 @@
+"""The omega test (pcsf_score_omega) on the GPU against the oracle's
+restatement of OmegaModel.score (src/OmegaModel.ml:160-219; paths relative to
+the mlin/PhyloCSF tree).
+
+With FP64 likelihoods the library's lock-step kr_map follows the oracle's
+maximiser (find_init, GSL Brent, three rounds of coordinate ascent), so the
+scores agree to 1e-5 dB and the estimates to 1e-5 (Brent's last steps are
+sqrt(eps)-sized, and on a flat optimum a comparison there can fall either way
+on 1e-12-level differences of the two eigensolvers; six chained maximisations
+carry such a difference along), on simulated regions and
+on the reference's example alignments, for the default H1 (omega 0.2, sigma
+0.01) and another one.  The MFMA precisions change the likelihoods by their
+rounding only; the maximiser may then stop elsewhere inside its 1% intervals,
+so they are held to 0.05 dB.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import phylocsf_amd as P  # noqa: E402
+from oracle import cli_oracle as CO  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return P.Context(0)
+
+
+def _sim_regions(paramset, lengths, seed):
+    """Regions simulated under the coding and noncoding ECMs (any codon
+    alignments will do for the omega test)."""
+    m = P.PhyloCSFModel.from_paramset(None, paramset)
+    rng = np.random.default_rng(seed)
+    R = len(lengths)
+    return m.simulate(lengths, rng.integers(0, 2, R), rng.uniform(0.3, 3.0, R), gap_p=0.1,
+                      seed=seed)
+
+
+def _check(res, codes, off, otree, h1, tol_rho=1e-5, tol_db=1e-5):
+    for r in range(len(off) - 1):
+        ref = O.omega_score(otree, codes[off[r]:off[r + 1]], *h1)
+        assert res["status"][r] == 0, r
+        assert abs(res["score"][r] - ref["score"]) <= tol_db, (r, res["score"][r], ref["score"])
+        for h, key in ((0, "H0"), (1, "H1")):
+            assert abs(res["rho"][r, h] - ref["rho_" + key]) <= tol_rho * ref["rho_" + key], (r, h)
+            assert abs(res["kappa"][r, h] - ref["kappa_" + key]) <= tol_rho * ref["kappa_" + key]
+            assert abs(res["lpr"][r, h] - ref["lpr_" + key]) <= 1e-9 * abs(ref["lpr_" + key])
+
+
+@pytest.mark.parametrize("paramset", ["12flies", "29mammals"])
+def test_omega_matches_oracle(ctx, params_dir, paramset):
+    _say("omega test vs oracle, " + paramset)
+    tree = P.newick_tree(open(os.path.join(params_dir, paramset + ".nh")).read())
+    otree = O.newick_tree(open(os.path.join(params_dir, paramset + ".nh")).read())
+    codes, off = _sim_regions(paramset, np.array([40, 1, 75, 12]), 3)
+    m = P.PhyloCSFModel.tree_only(ctx, tree)
+    b = P.Batch(m, codes, off)
+    for h1 in ((0.2, 0.01), (0.5, 0.3)):
+        res = P.score_omega(b, *h1, precision=P.PCSF_FP64)
+        _check(res, codes, off, otree, h1)
+
+
+def test_omega_example_alignments(ctx, params_dir, examples_dir):
+    """The reference's examples through the omega strategy's per-region path
+    (frames and ORFs as the CLI makes them)."""
+    for paramset, fa, kw in (("12flies", "tal-AA.fa", {}),
+                             ("29mammals", "ALDH2.exon5.fa", {"frames": 3})):
+        _say("omega test, " + fa)
+        tree = P.newick_tree(open(os.path.join(params_dir, paramset + ".nh")).read())
+        otree = O.newick_tree(open(os.path.join(params_dir, paramset + ".nh")).read())
+        sp, seqs = CO.input_mfa(open(os.path.join(examples_dir, fa)).read())
+        which = {s: i for i, s in enumerate(sp)}
+        leaf_ord = [which.get(tree.labels[i]) for i in range(tree.nl)]
+        rgns = CO.candidate_regions(seqs[0], kw.get("frames", 1), "AsIs", 25)
+        rc_seqs = [CO.revcomp(s) for s in seqs]
+        blocks = [CO.pleaves(rc_seqs if rc else seqs, leaf_ord, lo, hi) for rc, lo, hi in rgns]
+        m = P.PhyloCSFModel.tree_only(ctx, tree)
+        b = m.batch(blocks)
+        res = P.score_omega(b, precision=P.PCSF_FP64)
+        codes = np.concatenate(blocks)
+        off = np.cumsum([0] + [x.shape[0] for x in blocks])
+        _check(res, codes, off, otree, (0.2, 0.01))
+
+
+def test_omega_mfma_precisions(ctx, params_dir):
+    _say("omega test at FP32 and split FP16 vs FP64")
+    codes, off = _sim_regions("29mammals", np.array([60, 120, 33]), 8)
+    tree = P.newick_tree(open(os.path.join(params_dir, "29mammals.nh")).read())
+    b = P.Batch(P.PhyloCSFModel.tree_only(ctx, tree), codes, off)
+    r64 = P.score_omega(b, precision=P.PCSF_FP64)
+    for prec in (P.PCSF_FP32, P.PCSF_FP16S):
+        r = P.score_omega(b, precision=prec)
+        assert (r["status"] == 0).all()
+        d = np.abs(r["score"] - r64["score"])
+        _say("  precision %d: max |d score| %.3g dB" % (prec, d.max()))
+        assert d.max() <= 0.05
+
+
+def test_tree_only_model_refuses_ecm_entry_points(ctx, params_dir):
+    tree = P.newick_tree(open(os.path.join(params_dir, "12flies.nh")).read())
+    m = P.PhyloCSFModel.tree_only(ctx, tree)
+    b = P.Batch(m, np.zeros((5, tree.nl), np.uint8), np.array([0, 5]))
+    with pytest.raises(P.InvalidArgument):
+        b.forward(1.0, P.PCSF_FP32)
+    with pytest.raises(P.InvalidArgument):
+        P.score(b, "mle")
+    with pytest.raises(P.InvalidArgument):
+        m.transition(0, 1.0)
