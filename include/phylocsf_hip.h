@@ -50,6 +50,12 @@ extern "C" {
                         FP32 accumulation */
 #define PCSF_FP16 3  /* FP16 P and partial (scaled by 2^14), FP32 accumulation */
 #define PCSF_BF16 4  /* BF16 P and partial, FP32 accumulation */
+#define PCSF_FP64M 5 /* FP64 on matrix cores (v_mfma_f64_16x16x4f64): 64
+                        columns per workgroup, messages in FP64 as PCSF_FP64
+                        with another summation order; no tree-size limit.
+                        Not an MFMA precision in the sense above (no
+                        rescaling, no FP32 rounding): the maximisers treat it
+                        as PCSF_FP64, and take their final evaluations with it */
 
 typedef struct pcsf_ctx pcsf_ctx;
 typedef struct pcsf_model pcsf_model;

@@ -22,6 +22,7 @@ PCSF_FP32 = 1
 PCSF_FP16S = 2
 PCSF_FP16 = 3
 PCSF_BF16 = 4
+PCSF_FP64M = 5  # FP64 on matrix cores
 PCSF_N_VARIANTS = 7  # launch shapes of the fused kernel (pcsf_autotune)
 PCSF_NOALLOC_SCRATCH_BYTES = 32 << 20  # scratch of pcsf_forward_noalloc, per GPU
 

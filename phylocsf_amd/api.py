@@ -19,7 +19,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from ._lib import (PCSF_BF16, PCSF_FP16, PCSF_FP16S, PCSF_FP32, PCSF_FP64,  # noqa: F401
-                   PCSF_MARGINALIZE, PCSF_N_VARIANTS, PCSF_STRATEGY_FIXED, PCSF_STRATEGY_MLE,
+                   PCSF_FP64M, PCSF_MARGINALIZE, PCSF_N_VARIANTS, PCSF_STRATEGY_FIXED, PCSF_STRATEGY_MLE,
                    PCSF_STRATEGY_MLE_GRAD, InvalidArgument, lib, raise_for)
 
 DB = 10.0 / np.log(10.0)  # PhyloCSFModel.db (src/PhyloCSFModel.ml:113)

@@ -56,6 +56,10 @@ void dfree(T*& p) {
 
 bool rho_valid(double r) { return r > 0.0 && std::isfinite(r); }
 
+bool known_precision(int p) { return p >= PCSF_FP64 && p <= PCSF_FP64M; }
+// FP64 arithmetic throughout (plain or on matrix cores)
+bool fp64_precision(int p) { return p == PCSF_FP64 || p == PCSF_FP64M; }
+
 constexpr const char* kTreeOnly =
     "a tree-only model (made with n_models = 0) serves the omega test only";
 
@@ -279,6 +283,20 @@ struct SetModels {
   const double* lnpi;   // np * 64
 };
 
+// Node tiles of the FP64 matrix-core kernel (forward and outside pass): one
+// set per resident wave, allocated on first use.
+int ensure_adj_scratch(pcsf_batch* b) {
+  const pcsf_model* m = b->m;
+  pcsf_ctx* c = m->ctx;
+  const size_t need = adjoint_mfma_scratch_doubles(m, adjoint_mfma_grid(m));
+  if (b->adj_scratch >= need) return PCSF_OK;
+  dfree(b->d_adj_scratch);
+  b->adj_scratch = 0;
+  HIP_TRY(c, dalloc(&b->d_adj_scratch, need));
+  b->adj_scratch = need;
+  return PCSF_OK;
+}
+
 // Forward with one tree scale per (region, model): rho[r*nm+q].  active
 // (optional, r*nm+q) restricts the work to the listed pairs; results of the
 // others are left as they were.  Regions go through in chunks whose
@@ -290,10 +308,14 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
   const int nm = m->nm;
   int rc = ensure_aligned(b);
   if (rc) return rc;
-  const bool fp64 = precision == PCSF_FP64;
+  const bool fp64 = fp64_precision(precision);
   const int fmt = fp64 ? kFragF32 : frag_format(precision);
   const size_t budget = (size_t)8 << 30;
   const bool own = sm != nullptr;
+  if (precision == PCSF_FP64M) {
+    rc = ensure_adj_scratch(b);
+    if (rc) return rc;
+  }
   int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, fp64, false, own) / nm * nm);
   cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
   if (b->rs.cap < cap || (fp64 && !b->rs.fp64) || (own != (b->rs.d_eVT != nullptr))) {
@@ -348,9 +370,12 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
     }
     b->t_pbuild += tp;
     if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
-    hipError_t e = fp64 ? launch_prune_fp64_regions(m, b, ch, c->stream)
-                        : launch_prune_mfma_regions(m, b, precision, variant, ch, c->stream);
-    if (e == hipErrorInvalidValue && fp64)
+    hipError_t e =
+        (precision == PCSF_FP64M)
+            ? launch_forward_fp64m(m, b, ch, b->d_adj_scratch, adjoint_mfma_grid(m), c->stream)
+        : fp64 ? launch_prune_fp64_regions(m, b, ch, c->stream)
+               : launch_prune_mfma_regions(m, b, precision, variant, ch, c->stream);
+    if (e == hipErrorInvalidValue && precision == PCSF_FP64)
       return set_err(c, PCSF_EINVAL, "FP64 kernel: tree workspace exceeds 64 KiB of LDS");
     HIP_TRY(c, e);
     if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[1], c->stream));
@@ -414,12 +439,15 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
     float tp = 0.f;
     rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, kFragF32, &tp);
     if (rc) return rc;
-    hipError_t e = tangent ? launch_prune_tangent_regions(m, b, ch, c->stream)
-                           : launch_adjoint_fp64(
-                                 m, b, ch, d_gbr ? d_gbr + (size_t)r0 * nm * m->nbr : nullptr,
-                                 c->stream);
-    if (e == hipErrorInvalidValue && !tangent)
-      return set_err(c, PCSF_EINVAL, "adjoint kernel: tree workspace exceeds 64 KiB of LDS");
+    hipError_t e;
+    if (tangent) {
+      e = launch_prune_tangent_regions(m, b, ch, c->stream);
+    } else {
+      rc = ensure_adj_scratch(b);
+      if (rc) return rc;
+      e = launch_adjoint_mfma(m, b, ch, d_gbr ? d_gbr + (size_t)r0 * nm * m->nbr : nullptr,
+                              b->d_adj_scratch, adjoint_mfma_grid(m), c->stream);
+    }
     HIP_TRY(c, e);
     HIP_TRY(c, launch_region_reduce_aligned(b, ch, b->d_gloglik, b->d_grad, c->stream));
   }
@@ -949,6 +977,7 @@ void pcsf_batch_destroy(pcsf_batch* b) {
   dfree(b->d_acolelpr);
   dfree(b->d_gloglik);
   dfree(b->d_grad);
+  dfree(b->d_adj_scratch);
   free_psets(b->rs);
   for (auto& ev : b->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -981,7 +1010,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   pcsf_ctx* c = m->ctx;
   if (m->tree_only) return set_err(c, PCSF_EINVAL, kTreeOnly);
   (void)save_for_backward;
-  if (precision < PCSF_FP64 || precision > PCSF_BF16)
+  if (!known_precision(precision))
     return set_err(c, PCSF_EINVAL, "pcsf_forward: unknown precision");
   if (rho_per_region) {
     const size_t nr = (size_t)b->nreg * m->nm;
@@ -1008,10 +1037,14 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
                      "CamlPaml.P14n.instantiate_tree: domain violation on variable 0");
   HIP_TRY(c, hipSetDevice(c->device));
   float tp = 0.f;
-  const int fmt = precision == PCSF_FP64 ? (m->fixed.fmt < 0 ? kFragF32 : m->fixed.fmt)
-                                         : frag_format(precision);
+  const int fmt = fp64_precision(precision) ? (m->fixed.fmt < 0 ? kFragF32 : m->fixed.fmt)
+                                            : frag_format(precision);
   int rc = ensure_transitions(m, r.data(), fmt, &tp);
   if (rc) return rc;
+  if (precision == PCSF_FP64M && b->nreg > 0) {
+    // region-aligned columns, the model's sets for every region
+    if ((rc = ensure_aligned(b)) || (rc = ensure_adj_scratch(b))) return rc;
+  }
   b->t_pbuild = tp;
   b->last_rho.resize((size_t)b->nreg * m->nm);
   b->last_precision = precision;
@@ -1019,7 +1052,17 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
     for (int q = 0; q < m->nm; ++q) b->last_rho[(size_t)i * m->nm + q] = r[q];
   if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
   hipError_t e;
-  if (precision == PCSF_FP64) {
+  RegionChunk all;
+  all.r0 = 0;
+  all.nreg = b->nreg;
+  all.block0 = 0;
+  all.nblocks = b->al.ld / kAlignCols;
+  all.sets = &m->fixed;
+  if (precision == PCSF_FP64M) {
+    e = b->nreg > 0 ? launch_forward_fp64m(m, b, all, b->d_adj_scratch, adjoint_mfma_grid(m),
+                                           c->stream)
+                    : hipSuccess;
+  } else if (precision == PCSF_FP64) {
     e = launch_prune_fp64(m, b, c->stream);
     if (e == hipErrorInvalidValue)
       return set_err(c, PCSF_EINVAL, "FP64 kernel: tree workspace exceeds 64 KiB of LDS");
@@ -1030,7 +1073,12 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   }
   HIP_TRY(c, e);
   if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[1], c->stream));
-  HIP_TRY(c, launch_region_reduce(b, c->stream));
+  if (precision == PCSF_FP64M) {
+    if (b->nreg > 0) HIP_TRY(c, launch_region_reduce_aligned(b, all, b->d_loglik, b->d_elpr,
+                                                              c->stream));
+  } else {
+    HIP_TRY(c, launch_region_reduce(b, c->stream));
+  }
   if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[2], c->stream));
   const size_t nres = (size_t)b->nreg * m->nm;
   if (loglik && nres)
@@ -1051,7 +1099,11 @@ int pcsf_forward_noalloc(pcsf_model* m, const uint8_t* d_codes, int64_t n_cols,
   if (!m || !m->ctx || !rho) return PCSF_EINVAL;
   pcsf_ctx* c = m->ctx;
   if (m->tree_only) return set_err(c, PCSF_EINVAL, kTreeOnly);
-  if (precision < PCSF_FP64 || precision > PCSF_BF16)
+  if (precision == PCSF_FP64M)
+    return set_err(c, PCSF_EINVAL,
+                   "pcsf_forward_noalloc: PCSF_FP64M keeps one FP64 tile per tree node and "
+                   "column in device memory it allocates; use PCSF_FP64 here");
+  if (!known_precision(precision))
     return set_err(c, PCSF_EINVAL, "pcsf_forward_noalloc: unknown precision");
   if (n_cols < 0 || n_regions < 0 || (n_cols > 0 && !d_codes) || !d_region_offsets ||
       (n_regions > 0 && (!d_loglik || !d_elpr)))
@@ -1282,7 +1334,7 @@ int score_mle(pcsf_batch* b, int precision, std::vector<PairState>& st) {
     // less than that noise, and GSL's comparison of f(x +- tol) with f(x)
     // would be decided by rounding; such steps are 1e-5 |x| instead, still
     // far inside the maximiser's 1% stopping interval.
-    if (precision != PCSF_FP64) s.br.rel_tol = 1e-5;
+    if (!fp64_precision(precision)) s.br.rel_tol = 1e-5;
     s.br.propose();
     s.pending = s.br.u;
     s.phase = PairState::BRENT;
@@ -1391,14 +1443,14 @@ int score_mle(pcsf_batch* b, int precision, std::vector<PairState>& st) {
       }
     }
   }
-  if (precision != PCSF_FP64) {
+  if (!fp64_precision(precision)) {
     // the reference's final `f x` in FP64: the search ran on the MFMA
     // precision's likelihoods, the score is FP64's at its estimate
     for (size_t k = 0; k < np; ++k) {
       act[k] = st[k].status == PCSF_OK;
       rho[k] = act[k] ? st[k].res_x : 1.0;
     }
-    rc = forward_regions(b, rho.data(), act.data(), PCSF_FP64);
+    rc = forward_regions(b, rho.data(), act.data(), PCSF_FP64M);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(E.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost));
@@ -1431,7 +1483,7 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   // of it for the FP64 and the FP32-gradient tolerances.  FP32 gradients are
   // noisy at the 1e-3 level near the root (a few thousand nats of lpr), so
   // asking them for more than 1e-4 only buys bisection steps.
-  const double tol = (precision == PCSF_FP64) ? 1e-6 : 1e-4;
+  const double tol = fp64_precision(precision) ? 1e-6 : 1e-4;
   // Per pair: Newton steps on h = d lpr / d ln rho with h' from the last two
   // evaluations (secant), inside the bracket [lo, hi] that the signs of h
   // leave for the maximum; bisection when a step leaves the bracket or the
@@ -1454,7 +1506,7 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   std::vector<uint8_t> act(np, 1);
   auto eval = [&]() -> int {
     // FP64: the outside-pass adjoint; MFMA precisions: the FP32 tangent pass
-    int rc = adjoint_regions(b, rho.data(), act.data(), nullptr, precision != PCSF_FP64);
+    int rc = adjoint_regions(b, rho.data(), act.data(), nullptr, !fp64_precision(precision));
     if (rc) return rc;
     HIP_TRY(c, hipMemcpy(L.data(), b->d_gloglik, np * 8, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(D.data(), b->d_grad, np * 8, hipMemcpyDeviceToHost));
@@ -1482,7 +1534,7 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
       }
       // h = 0, or (FP32 gradients) below their rounding noise, ~1e-8 |lpr|:
       // the root is as close as these gradients can place it
-      if (h == 0.0 || (precision != PCSF_FP64 && std::fabs(h) <= 1e-7 * std::fabs(L[k]))) {
+      if (h == 0.0 || (!fp64_precision(precision) && std::fabs(h) <= 1e-7 * std::fabs(L[k]))) {
         s.done = true;
         continue;
       }
@@ -1537,7 +1589,7 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   // of a few thousand codons, ~3e-3 dB) would otherwise reach the score
   for (size_t k = 0; k < np; ++k) rho[k] = std::exp(g[k].x);
   std::vector<uint8_t> all(np, 1);
-  int rc = forward_regions(b, rho.data(), all.data(), PCSF_FP64);
+  int rc = forward_regions(b, rho.data(), all.data(), PCSF_FP64M);
   if (rc) return rc;
   HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
   HIP_TRY(c, hipMemcpy(D.data(), b->d_elpr, np * 8, hipMemcpyDeviceToHost));
@@ -1577,7 +1629,7 @@ int score_omega(pcsf_batch* b, int precision, double omega_h1, double sigma_h1,
   pcsf_ctx* c = m->ctx;
   const int R = b->nreg;
   const size_t np = (size_t)R * 2;
-  const double rel_tol = precision == PCSF_FP64 ? 1.4901161193847656e-08 : 1e-5;
+  const double rel_tol = fp64_precision(precision) ? 1.4901161193847656e-08 : 1e-5;
   // F3x4 settings per region, from the batch's codes
   std::vector<uint8_t> codes((size_t)b->ncols * m->nl);
   int rc = pcsf_batch_codes(b, codes.data(), nullptr);
@@ -1711,7 +1763,7 @@ int score_omega(pcsf_batch* b, int precision, double omega_h1, double sigma_h1,
         st[k + 1].waiting = false;
       }
   }
-  if (precision != PCSF_FP64) {
+  if (!fp64_precision(precision)) {
     // each hypothesis's lpr at its estimates in FP64 (the fits ran on the
     // MFMA precision's likelihoods): the last kappa step's f = lpr_kappa + lpr
     std::vector<size_t> need;
@@ -1725,7 +1777,7 @@ int score_omega(pcsf_batch* b, int precision, double omega_h1, double sigma_h1,
         st[k].status = PCSF_EFAIL;
         act[k] = 0;
       }
-    rc = forward_regions(b, rho.data(), act.data(), PCSF_FP64, &sm);
+    rc = forward_regions(b, rho.data(), act.data(), PCSF_FP64M, &sm);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < np; ++k)
@@ -1797,7 +1849,7 @@ int pcsf_score_omega(pcsf_batch* b, double omega_h1, double sigma_h1, int precis
   pcsf_ctx* c = m->ctx;
   if (m->nm != 2)
     return set_err(c, PCSF_EINVAL, "pcsf_score_omega: needs a model with two hypothesis slots");
-  if (precision < PCSF_FP64 || precision > PCSF_BF16)
+  if (!known_precision(precision))
     return set_err(c, PCSF_EINVAL, "pcsf_score_omega: unknown precision");
   if (!(omega_h1 >= 0.0) || !(sigma_h1 >= 0.0) || !std::isfinite(omega_h1) ||
       !std::isfinite(sigma_h1))
@@ -1842,7 +1894,7 @@ int pcsf_backward(pcsf_batch* b, const double* grad_loglik, double* grad_rho,
   }
   // an MFMA-precision forward without per-branch gradients is differentiated
   // by the FP32 tangent pass; everything else by the FP64 outside pass
-  const bool tangent = b->last_precision != PCSF_FP64 && !d_gbr;
+  const bool tangent = !fp64_precision(b->last_precision) && !d_gbr;
   int rc = adjoint_regions(b, b->last_rho.data(), nullptr, d_gbr, tangent);
   std::vector<double> dl(np), gb;
   if (!rc && np) {

@@ -12,9 +12,11 @@
 // still written alignment by alignment, in input order.
 //
 // Extra options (no counterpart in the reference):
-//   --precision=fp64|fp32|fp16s   arithmetic of the pruning (default fp64: the
-//                                 reference's digits; fp32 and split-fp16 agree
-//                                 with it to 1e-3 dB per region at fixed scales)
+//   --precision=fp64|fp64m|fp32|fp16s   arithmetic of the pruning (default
+//                                 fp64: the reference's digits; fp64m the same
+//                                 FP64 arithmetic on matrix cores; fp32 and
+//                                 split-fp16 agree with it to 1e-3 dB per
+//                                 region at fixed scales)
 //   --device=N                    GPU index (default 0)
 //   --batchBytes=N                leaf-code bytes packed per GPU call (default 1 GiB)
 // The omega strategy (src/OmegaModel.ml) runs every candidate region's two
@@ -127,7 +129,7 @@ const char* kUsage =
     "    --aa                       include amino acid translation in output\n"
     "    --debug                    print extra information about parameters and errors\n\n"
     "  GPU:\n"
-    "    --precision=fp64|fp32|fp16s  arithmetic of the pruning kernel (default fp64)\n"
+    "    --precision=fp64|fp64m|fp32|fp16s  arithmetic of the pruning kernel (default fp64)\n"
     "    --device=N                 GPU index (default 0)\n"
     "    --batchBytes=N             leaf-code bytes packed per GPU call (default 1 GiB)\n";
 
@@ -234,9 +236,10 @@ Options parse_options(int argc, char** argv) {
     } else if (name == "precision") {
       const std::string v = lower(val);
       if (v == "fp64") o.precision = PCSF_FP64;
+      else if (v == "fp64m") o.precision = PCSF_FP64M;
       else if (v == "fp32") o.precision = PCSF_FP32;
       else if (v == "fp16s") o.precision = PCSF_FP16S;
-      else usage_error("option --precision: fp64, fp32 or fp16s");
+      else usage_error("option --precision: fp64, fp64m, fp32 or fp16s");
     } else if (name == "device") {
       o.device = to_int(val, a);
     } else if (name == "batchBytes") {

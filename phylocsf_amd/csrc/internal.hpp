@@ -155,13 +155,16 @@ struct pcsf_batch {
   // last forward's d_loglik / d_elpr stay as they were
   double* d_gloglik = nullptr;      // nreg * nm: lpr at the differentiated scales
   double* d_grad = nullptr;         // nreg * nm: d lpr / d rho
+  double* d_adj_scratch = nullptr;  // FP64 outside pass: node tiles per resident wave
+  size_t adj_scratch = 0;           // its size in doubles
 };
 
 // Per-region-scale launch description (one chunk of regions).
 struct RegionChunk {
   int32_t r0 = 0, nreg = 0;         // regions [r0, r0 + nreg)
   int64_t block0 = 0, nblocks = 0;  // aligned blocks covering them
-  const PSets* sets = nullptr;      // set (r - r0) * nm + q
+  const PSets* sets = nullptr;      // set (r - r0) * nm + q; set q when `sets` is the
+                                    // model's own (fixed) sets, shared by every region
 };
 
 // ---- kernel launchers (kernels/*.hip) --------------------------------------
@@ -182,8 +185,40 @@ hipError_t launch_align_codes(const pcsf_batch* b, hipStream_t s);
 hipError_t launch_frames(const uint8_t* nuc, int64_t nlen, const int64_t* d_roff,
                          const int64_t* d_foff, int32_t nreg, int frames, int nl, uint8_t* dst,
                          int64_t ld, hipStream_t s);
-hipError_t launch_adjoint_fp64(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
-                               double* grad_branch, hipStream_t s);
+// The outside pass on FP64 matrix cores (kernels/adjoint_mfma.hip): region-
+// aligned blocks of 64 columns per workgroup; scratch holds one 64 x 16 FP64
+// tile per internal node for each of the grid's waves.
+struct AdjointArgs {
+  const uint8_t* codes;
+  int64_t ld, ald;           // leading dimensions of codes and of the per-column outputs
+  int64_t c0, nblocks;       // first column and block count of the chunk
+  int n, nl, nm, nbr;
+  const int32_t *lc, *rc;
+  const double* P64T;        // sets: P^T per branch (P64T[b * 64 + a] = P[a][b])
+  const double* QT;          // per model: Q^T
+  const double* pi;          // per model, or per set with prior_per_set
+  const double* lnpi;        // its logarithm
+  int prior_per_set;
+  const double* branch;      // nbr
+  const int32_t* block_region;
+  const int64_t* astart;
+  const int64_t* off;
+  const uint8_t* active;
+  int r0;
+  int shared_sets;           // 1: set q for every region (the model's fixed sets)
+  double* logz;              // nm x ald
+  double* coldrho;           // nm x ald: d ln z / d rho per column
+  double* grad_branch;       // (region - r0) x nm x nbr, added to (may be null)
+  double* scratch;
+};
+int adjoint_mfma_grid(const pcsf_model* m);
+size_t adjoint_mfma_scratch_doubles(const pcsf_model* m, int grid);
+hipError_t launch_adjoint_mfma(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
+                               double* grad_branch, double* scratch, int grid, hipStream_t s);
+// the same kernel's inside pass only: ln z and the root's expected ln pi per
+// column into the batch's aligned per-column buffers (precision PCSF_FP64M)
+hipError_t launch_forward_fp64m(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
+                                double* scratch, int grid, hipStream_t s);
 hipError_t launch_prune_tangent_regions(const pcsf_model* m, const pcsf_batch* b,
                                         const RegionChunk& ch, hipStream_t s);
 // sums over each region of the chunk into loglik / elpr (nreg * nm)

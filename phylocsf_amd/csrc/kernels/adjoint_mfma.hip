@@ -1,0 +1,328 @@
+// adjoint_mfma.hip -- the outside (preorder) pass on FP64 matrix cores:
+// d ln z / d t_br per region and branch, and d ln z / d rho per column.
+//
+// The gradient the reference's PhyloEM defines (lib/CamlPaml/PhyloEM.ml:
+// 140-182): d ELL / d t_br = sum_ab xi_br[a,b] (dP/dt)_ab / P_ab with the
+// branch posteriors xi of PhyloLik.add_branch_posteriors (PhyloLik.ml:
+// 140-174) and dP/dt = Q P(t) (Q.ml:260-275); by the Fisher identity it is
+// d ln z / d t_br, and with every branch scaled by rho, d ln z / d rho =
+// sum_br b_br d ln z / d t_br (PhyloEM.ml:162-182).  For the branch into
+// child c of node p with sibling s, with messages m_c = P_c alpha_c:
+//   w_c = beta_p o m_s,   d ln z / d t_c = w_c . (Q m_c) / z,
+//   beta_c = P_c^T w_c    (PhyloLik.ensure_beta, PhyloLik.ml:96-120),
+// beta_root = pi (PhyloLik.ml:62), alpha_p = m_l o m_r (:78-90).
+//
+// Layout for CDNA4: a wave owns 16 columns as 64 x 16 FP64 tiles (lane
+// j + 16 g holds column j, rows 16 mo + 4 v + g in element v of r[mo]: the
+// accumulator layout of v_mfma_f64_16x16x4f64, which is also its B operand
+// for the K-steps taken in natural order), and the four waves of a workgroup
+// (64 columns: one region-aligned block) walk the tree in lock step sharing
+// one 64 x 64 FP64 matrix in LDS, stored in the MFMA's A-fragment order
+// (element (mo, s, lane) at ((16 mo + s) 64 + lane): conflict-free 8-byte
+// reads), with Q resident beside it.  The inside pass stores each internal
+// node's message to its parent in a per-wave global slot; the outside pass
+// reads the two messages of each node, overwrites them with the children's
+// beta, and so needs no more memory.  Per-branch sums are reduced over the
+// wave's 16 columns, then over the workgroup in LDS, and added to the region's
+// total with one FP64 atomic per (workgroup, branch).  No rescaling, as the
+// reference (FP64 range).
+//
+// The same kernel without its outside pass (kFwd) is the FP64 forward on
+// matrix cores (precision PCSF_FP64M): per column ln z and the root
+// posterior's expected ln pi (PhyloLik.ml:127-138 at the root, the elpr_anc
+// term of PhyloCSF.ml), with the priors per model or per set (the omega
+// test's own models).
+#include "../internal.hpp"
+
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+struct Tile {
+  f64x4 r[4];
+};
+
+constexpr int kWaves = 4;                    // waves per workgroup
+constexpr int kCols = 16 * kWaves;           // columns per workgroup (= kAlignCols)
+static_assert(kCols == kAlignCols, "a workgroup covers one region-aligned block");
+constexpr int kTileDoubles = 64 * 16;        // one 64 x 16 tile
+
+// A 64 x 64 matrix M into LDS in A-fragment order: M[row][col] goes to
+// ((row / 16) * 16 + col / 4) * 64 + (row % 16) + 16 (col % 4).  `src`
+// gives M either as M[row][col] = src[row * 64 + col] (by_rows) or as
+// M[row][col] = src[col * 64 + row]; the global reads run in src order.
+__device__ __forceinline__ void stage(double* __restrict__ frag, const double* __restrict__ src,
+                                      bool by_rows, int tid) {
+#pragma unroll 4
+  for (int e = tid; e < 4096; e += 64 * kWaves) {
+    const int hi = e >> 6, lo = e & 63;
+    const int row = by_rows ? hi : lo, col = by_rows ? lo : hi;
+    frag[((row >> 4) * 16 + (col >> 2)) * 64 + (row & 15) + 16 * (col & 3)] = src[e];
+  }
+}
+
+// acc = M B for the staged M and this wave's tile B
+__device__ __forceinline__ Tile gemm(const double* __restrict__ frag, const Tile& b, int lane) {
+  Tile acc;
+#pragma unroll
+  for (int mo = 0; mo < 4; ++mo) {
+    f64x4 c = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+      c = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[(mo * 16 + s) * 64 + lane], b.r[s >> 2][s & 3],
+                                               c, 0, 0, 0);
+    acc.r[mo] = c;
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void store(double* __restrict__ slot, const Tile& t, int lane) {
+#pragma unroll
+  for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) slot[(mo * 4 + v) * 64 + lane] = t.r[mo][v];
+}
+
+__device__ __forceinline__ Tile load(const double* __restrict__ slot, int lane) {
+  Tile t;
+#pragma unroll
+  for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) t.r[mo][v] = slot[(mo * 4 + v) * 64 + lane];
+  return t;
+}
+
+// Every matrix product below runs in wave-uniform control flow: the A
+// fragments come from LDS through all 64 lanes.
+//
+// the message of leaf c: column `code` of P_c (PT[code * 64 + row] =
+// P[row][code]), or ones for `Marginalize
+__device__ __forceinline__ Tile leaf_message(const double* __restrict__ PT, int code, int grp) {
+  Tile t;
+#pragma unroll
+  for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      t.r[mo][v] = (code == PCSF_MARGINALIZE) ? 1.0 : PT[code * 64 + 16 * mo + 4 * v + grp];
+  return t;
+}
+
+// sum over the 64 rows of this lane's column of x o y (all four lane groups
+// of the column end with it)
+__device__ __forceinline__ double column_dot(const Tile& x, const Tile& y) {
+  double s = 0.0;
+#pragma unroll
+  for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) s = fma(x.r[mo][v], y.r[mo][v], s);
+  s += __shfl_xor(s, 16);
+  s += __shfl_xor(s, 32);
+  return s;
+}
+
+template <bool kFwd>
+__global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArgs p) {
+  __shared__ double Qf[kFwd ? 1 : 4096];  // Q of the task's model, A-fragment order
+  __shared__ double Af[4096];             // the current P_c or P_c^T
+  extern __shared__ double gsum[];        // nbr: per-branch sums of the task's workgroup
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, grp = lane >> 4;
+  const int j = lane & 15;
+  const int n = p.n, nl = p.nl, nm = p.nm, nbr = p.nbr;
+  // this wave's message / beta slots, one tile per internal node
+  double* slots = p.scratch + ((size_t)blockIdx.x * kWaves + wave) * (n - nl) * kTileDoubles;
+  const int64_t ntask = p.nblocks * nm;
+  int staged_q = -1;
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const int64_t blk = task / nm;
+    const int q = (int)(task - blk * nm);
+    const int64_t b0 = p.c0 + blk * kCols;
+    const int r = p.block_region[b0 / kAlignCols];
+    if (r < 0) continue;  // padding block (uniform over the workgroup)
+    const int set = p.shared_sets ? q : (r - p.r0) * nm + q;
+    if (!p.active[set]) continue;
+    const int64_t col = b0 + wave * 16 + j;
+    const bool valid = col - p.astart[r] < p.off[r + 1] - p.off[r];
+    const double* PT = p.P64T + (size_t)set * nbr * 4096;  // PT[br][b * 64 + a] = P[a][b]
+    __syncthreads();  // the previous task is done with Qf, Af and gsum
+    if (!kFwd) {
+      if (q != staged_q) {
+        stage(Qf, p.QT + (size_t)q * 4096, false, tid);  // QT[k * 64 + a] = Q[a][k]
+        staged_q = q;
+      }
+      for (int br = tid; br < nbr; br += 64 * kWaves) gsum[br] = 0.0;
+      __syncthreads();  // Qf and gsum ready (a two-leaf tree has no barrier before its use)
+    }
+    // ---- inside pass (PhyloLik.ml:78-92), messages to the slots
+    double z = 0.0;
+    for (int i = nl; i < n; ++i) {
+      const int l = p.lc[i], rr = p.rc[i];
+      Tile a = (l < nl) ? leaf_message(PT + (size_t)l * 4096, p.codes[(size_t)l * p.ld + col], grp)
+                        : load(slots + (size_t)(l - nl) * kTileDoubles, lane);
+      const Tile mr = (rr < nl)
+                          ? leaf_message(PT + (size_t)rr * 4096, p.codes[(size_t)rr * p.ld + col], grp)
+                          : load(slots + (size_t)(rr - nl) * kTileDoubles, lane);
+#pragma unroll
+      for (int mo = 0; mo < 4; ++mo) a.r[mo] *= mr.r[mo];  // alpha_i
+      if (i == n - 1) {
+        const int pq = p.prior_per_set ? set : q;  // one prior per model, or per set
+        Tile pr;
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) pr.r[mo][v] = p.pi[pq * 64 + 16 * mo + 4 * v + grp];
+        z = column_dot(a, pr);
+        if (kFwd) {  // sum_x (alpha_x pi_x / z) ln pi_x
+#pragma unroll
+          for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              a.r[mo][v] *= pr.r[mo][v];
+              pr.r[mo][v] = p.lnpi[pq * 64 + 16 * mo + 4 * v + grp];
+            }
+          const double e = column_dot(a, pr);
+          if (grp == 0 && valid) {
+            p.logz[(size_t)q * p.ald + col] = log(z);
+            p.coldrho[(size_t)q * p.ald + col] = (z != 0.0) ? e / z : 0.0;
+          }
+        }
+        break;
+      }
+      __syncthreads();  // Af free
+      stage(Af, PT + (size_t)i * 4096, false, tid);
+      __syncthreads();
+      store(slots + (size_t)(i - nl) * kTileDoubles, gemm(Af, a, lane), lane);
+    }
+    if (kFwd) continue;
+    const double zinv = (z != 0.0) ? 1.0 / z : 0.0;
+    // ---- outside pass, parents before children
+    double drho = 0.0;
+    for (int pn = n - 1; pn >= nl; --pn) {
+      const int kid[2] = {p.lc[pn], p.rc[pn]};
+      Tile m[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+        m[s] = (kid[s] < nl) ? leaf_message(PT + (size_t)kid[s] * 4096,
+                                            p.codes[(size_t)kid[s] * p.ld + col], grp)
+                             : load(slots + (size_t)(kid[s] - nl) * kTileDoubles, lane);
+      Tile bp;
+      if (pn == n - 1) {
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) bp.r[mo][v] = p.pi[q * 64 + 16 * mo + 4 * v + grp];
+      } else {
+        bp = load(slots + (size_t)(pn - nl) * kTileDoubles, lane);
+      }
+      // w_c = beta_p o m_sibling
+      Tile w[2];
+#pragma unroll
+      for (int mo = 0; mo < 4; ++mo) {
+        w[0].r[mo] = bp.r[mo] * m[1].r[mo];
+        w[1].r[mo] = bp.r[mo] * m[0].r[mo];
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int c = kid[s];
+        // d ln z / d t_c = w_c . (Q m_c) / z for this column.  The product
+        // runs on every lane: an MFMA takes its A rows from all 64 lanes, so
+        // a lane of a padding column that skipped the fragment loads would
+        // hand stale rows to the valid columns.
+        const double wqm = column_dot(w[s], gemm(Qf, m[s], lane));
+        const double g = valid ? wqm * zinv : 0.0;
+        drho = fma(p.branch[c], g, drho);
+        if (p.grad_branch) {
+          double t = g;  // over the wave's 16 columns
+          t += __shfl_xor(t, 1);
+          t += __shfl_xor(t, 2);
+          t += __shfl_xor(t, 4);
+          t += __shfl_xor(t, 8);
+          if (lane == 0) atomicAdd(&gsum[c], t);
+        }
+        if (c >= nl) {  // beta_c = P_c^T w_c into c's slot
+          __syncthreads();
+          stage(Af, PT + (size_t)c * 4096, true, tid);  // P^T[b][a] = PT[b * 64 + a]
+          __syncthreads();
+          store(slots + (size_t)(c - nl) * kTileDoubles, gemm(Af, w[s], lane), lane);
+        }
+      }
+    }
+    if (grp == 0) {
+      p.logz[(size_t)q * p.ald + col] = log(z);
+      p.coldrho[(size_t)q * p.ald + col] = drho;
+    }
+    if (p.grad_branch) {
+      __syncthreads();
+      double* gb = p.grad_branch + ((size_t)(r - p.r0) * nm + q) * nbr;
+      for (int br = tid; br < nbr; br += 64 * kWaves)
+        if (gsum[br] != 0.0) atomicAdd(gb + br, gsum[br]);
+    }
+  }
+}
+
+}  // namespace
+
+size_t adjoint_mfma_scratch_doubles(const pcsf_model* m, int grid) {
+  return (size_t)grid * kWaves * (m->n - m->nl) * kTileDoubles;
+}
+
+int adjoint_mfma_grid(const pcsf_model* m) { return m->ctx->num_cus * 2; }
+
+namespace {
+
+AdjointArgs make_args(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
+                      double* scratch) {
+  AdjointArgs a;
+  a.codes = b->al.d_codes;
+  a.ld = b->al.ld;
+  a.ald = b->al.ld;
+  a.c0 = ch.block0 * kAlignCols;
+  a.nblocks = ch.nblocks;
+  a.n = m->n;
+  a.nl = m->nl;
+  a.nm = m->nm;
+  a.nbr = m->nbr;
+  a.lc = m->d_lc;
+  a.rc = m->d_rc;
+  a.P64T = ch.sets->d_P64T;
+  a.QT = m->d_QT;
+  a.prior_per_set = ch.sets->d_epi ? 1 : 0;
+  a.pi = a.prior_per_set ? ch.sets->d_epi : m->d_pi;
+  a.lnpi = a.prior_per_set ? ch.sets->d_elnpi : m->d_lnpi;
+  a.branch = m->d_branch;
+  a.block_region = b->al.d_block_region;
+  a.astart = b->al.d_astart;
+  a.off = b->d_offsets;
+  a.active = ch.sets->d_active;
+  a.r0 = ch.r0;
+  a.logz = b->d_alogz;
+  a.coldrho = b->d_acolelpr;
+  a.grad_branch = nullptr;
+  a.scratch = scratch;
+  a.shared_sets = ch.sets == &m->fixed ? 1 : 0;
+  return a;
+}
+
+}  // namespace
+
+hipError_t launch_adjoint_mfma(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
+                               double* grad_branch, double* scratch, int grid, hipStream_t s) {
+  AdjointArgs a = make_args(m, b, ch, scratch);
+  if (a.prior_per_set) return hipErrorInvalidValue;  // Q is per model
+  a.grad_branch = grad_branch;
+  const int64_t ntask = ch.nblocks * m->nm;
+  const int g = (int)std::min<int64_t>(ntask, grid);
+  if (g <= 0) return hipSuccess;
+  hipLaunchKernelGGL(adjoint_mfma_kernel<false>, dim3(g), dim3(64 * kWaves),
+                     m->nbr * sizeof(double), s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_forward_fp64m(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
+                                double* scratch, int grid, hipStream_t s) {
+  const AdjointArgs a = make_args(m, b, ch, scratch);
+  const int64_t ntask = ch.nblocks * m->nm;
+  const int g = (int)std::min<int64_t>(ntask, grid);
+  if (g <= 0) return hipSuccess;
+  hipLaunchKernelGGL(adjoint_mfma_kernel<true>, dim3(g), dim3(64 * kWaves), 0, s, a);
+  return hipGetLastError();
+}

@@ -135,8 +135,10 @@ hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& 
                                         double* loglik, double* elpr, hipStream_t s) {
   if (ch.nreg == 0) return hipSuccess;
   dim3 grid(ch.nreg, b->m->nm);
+  // the model's own sets (one per model, shared by every region) are all live
+  const uint8_t* active = (ch.sets == &b->m->fixed) ? nullptr : ch.sets->d_active;
   hipLaunchKernelGGL(region_reduce_aligned_kernel, grid, dim3(256), 0, s, b->d_alogz,
-                     b->d_acolelpr, b->d_offsets, b->al.d_astart, ch.sets->d_active, ch.r0,
+                     b->d_acolelpr, b->d_offsets, b->al.d_astart, active, ch.r0,
                      b->m->nm, b->al.ld, loglik, elpr);
   return hipGetLastError();
 }

@@ -4,6 +4,8 @@ Tolerances (the scoring path's contract):
   * FP64 plain kernel: |d score| <= 1e-6 dB, relative log-likelihood <= 1e-10
     (same arithmetic as the oracle up to summation order, FMA and the ~1e-15
     absolute error of two independent eigensolvers in small P entries).
+  * FP64 on matrix cores (PCSF_FP64M): the same bounds (FP64 products and
+    sums in another order).
   * FP32 MFMA kernel: |d score| <= 1e-3 dB and relative log-likelihood
     <= 1e-6 against the FP64 oracle.
 """
@@ -108,6 +110,7 @@ def test_other_trees_fixed(ctx, params_dir):
         ref = _oracle_scores(_oracle_model(params_dir, name), codes, off)
         b = P.Batch(m, codes, off)
         _check(P.score_fixed(b, P.PCSF_FP64), ref, TOL_DB_FP64, TOL_REL_FP64, name + "/FP64")
+        _check(P.score_fixed(b, P.PCSF_FP64M), ref, TOL_DB_FP64, TOL_REL_FP64, name + "/FP64M")
         _check(P.score_fixed(b, P.PCSF_FP32), ref, TOL_DB_FP32, TOL_REL_FP32, name + "/FP32")
 
 
@@ -132,6 +135,7 @@ def test_every_parameter_set_fixed(ctx, params_dir, name):
     ref = _oracle_scores(_oracle_model(params_dir, name), codes, off)
     b = P.Batch(m, codes, off)
     _check(P.score_fixed(b, P.PCSF_FP64), ref, TOL_DB_FP64, TOL_REL_FP64, name + "/FP64")
+    _check(P.score_fixed(b, P.PCSF_FP64M), ref, TOL_DB_FP64, TOL_REL_FP64, name + "/FP64M")
     _check(P.score_fixed(b, P.PCSF_FP32), ref, TOL_DB_FP32, TOL_REL_FP32, name + "/FP32")
     _check(P.score_fixed(b, P.PCSF_FP16S), ref, TOL_DB_FP32, TOL_REL_FP32, name + "/FP16S")
 
@@ -200,6 +204,7 @@ def test_edge_cases(ctx, params_dir):
     ref = _oracle_scores(om, codes, off)
     b = P.Batch(m, codes, off)
     for prec, tdb, trel in ((P.PCSF_FP64, TOL_DB_FP64, TOL_REL_FP64),
+                            (P.PCSF_FP64M, TOL_DB_FP64, TOL_REL_FP64),
                             (P.PCSF_FP32, TOL_DB_FP32, TOL_REL_FP32)):
         res = P.score_fixed(b, prec)
         _check(res, ref, tdb, trel, "edge/%d" % prec)
