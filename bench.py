@@ -135,7 +135,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="threads of the CPU baseline (the GPU host's share is 16 cores)")
     ap.add_argument("--regions", type=int, default=None, help="override region count")
-    ap.add_argument("--precision", choices=sorted(PRECISIONS) + ["fp64"], default=None,
+    ap.add_argument("--precision", choices=sorted(PRECISIONS) + ["fp64", "fp64m"], default=None,
                     help="GEMM precision of the batched MFMA path (fp16s and fp32 meet "
                          "the 1e-3 dB bound; fp16 and bf16 do not); default fp16s, and "
                          "fp32 for C4 (its configuration names FP32)")
@@ -147,9 +147,9 @@ def main():
     args = ap.parse_args()
     if args.precision is None:
         args.precision = "fp32" if args.workload == "C4" else "fp16s"
-    if args.precision == "fp64" and args.workload != "omega":
-        ap.error("--precision fp64 is for --workload omega (the other lines time the MFMA paths "
-                 "and report the plain FP64 kernel beside them)")
+    if args.precision in ("fp64", "fp64m") and args.workload != "omega":
+        ap.error("--precision fp64/fp64m are for --workload omega (the other lines time the MFMA "
+                 "paths and report the FP64 kernels beside them)")
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (regions shard "
               "across GPUs by design only); run with --gpus 1", file=sys.stderr)
@@ -386,10 +386,16 @@ def other_workload(args):
             res[strat] = (r, dt)
             log("%s: %.2f s, %.3e scored col/s, %d evaluations per region (mean)"
                 % (strat, dt, wl.n_cols / dt, r["n_evals"].mean()))
+        t1 = time.perf_counter()
         r64 = P.score(batch, "mle", P.PCSF_FP64)
+        dtr64 = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        r64m = P.score(batch, "mle", P.PCSF_FP64M)
+        dtr64m = time.perf_counter() - t1
         t1 = time.perf_counter()
         g64 = P.score(batch, "mle_grad", P.PCSF_FP64)
         dt64 = time.perf_counter() - t1
+        log("FP64 mle %.2f s, FP64M mle %.2f s, FP64 mle_grad %.2f s" % (dtr64, dtr64m, dt64))
         g, dtg = res["mle_grad"]
         b, dtb = res["mle"]
         # finite-difference check of d lpr / d rho at the gradient estimates
@@ -423,14 +429,34 @@ def other_workload(args):
                   "mle_vs_fp64_mle": float(np.max(np.abs(b["score"] - r64["score"])))},
               fp64_gradient_maximiser={"scored_columns_per_s": wl.n_cols / dt64,
                                        "mean_gradient_evaluations": float(g64["n_evals"].mean())},
+              fp64_reference_maximiser={
+                  "plain_kernel_scored_columns_per_s": wl.n_cols / dtr64,
+                  "matrix_core_kernel_scored_columns_per_s": wl.n_cols / dtr64m,
+                  "max_abs_delta_score_db_matrix_core_vs_plain":
+                      float(np.max(np.abs(r64m["score"] - r64["score"])))},
               lpr_gain_of_gradient_maximiser=float(np.max(g["lpr"] - r64["lpr"])),
               fd_rel_error_of_d_lpr_d_rho=fd_err,
               gradient_rel_error_vs_fp64_adjoint=grad_err)
         return
     # C5: every precision against FP64, fixed strategy; per-branch gradients at the mle
+    ctx.synchronize()
+    t1 = time.perf_counter()
     ll64, _ = batch.forward(1.0, P.PCSF_FP64)
+    dt64 = time.perf_counter() - t1
     s64 = P.DB * (ll64[:, 0] - ll64[:, 1])
-    per = {}
+    per = {"fp64": {"codon_columns_per_s": wl.n_cols / dt64}}
+    batch.forward(1.0, P.PCSF_FP64M)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    llm, _ = batch.forward(1.0, P.PCSF_FP64M)
+    dtm = time.perf_counter() - t1
+    sm = P.DB * (llm[:, 0] - llm[:, 1])
+    per["fp64m"] = {"codon_columns_per_s": wl.n_cols / dtm,
+                    "max_abs_delta_score_db": float(np.max(np.abs(sm - s64))),
+                    "max_rel_loglik_error": float(np.max(np.abs(llm - ll64) /
+                                                         np.maximum(1.0, np.abs(ll64))))}
+    log("fp64: %.3e col/s; fp64m: %.3e col/s, max |d score| %.3g dB"
+        % (wl.n_cols / dt64, wl.n_cols / dtm, per["fp64m"]["max_abs_delta_score_db"]))
     for pname in ("fp32", "fp16s", "fp16", "bf16"):
         pc = PRECISIONS[pname][0]
         batch.forward(1.0, pc)
@@ -498,15 +524,18 @@ def extra_workload(args, P, W, ctx):
         codes, off = sim.simulate(lengths, rng.integers(0, 2, R), np.ones(R), gap_p=0.1,
                                   n_p=0.01, seed=6, n_threads=16)
         batch = P.Batch(P.PhyloCSFModel.tree_only(ctx, tree), codes, off)
-        prec, dtype, _, _ = PRECISIONS[args.precision] if args.precision != "fp64" else \
-            (0, "fp64", None, None)
+        prec, dtype, _, _ = (PRECISIONS[args.precision] if args.precision in PRECISIONS else
+                             {"fp64": (0, "fp64", None, None),
+                              "fp64m": (5, "fp64 (matrix cores)", None, None)}[args.precision])
         t0 = time.perf_counter()
         r = P.score_omega(batch, precision=prec)
         dt = time.perf_counter() - t0
         log("omega (%s): %.2f s, %.3e scored col/s" % (args.precision, dt, off[-1] / dt))
         err = None
         if prec != 0:
+            t1 = time.perf_counter()
             r64 = P.score_omega(batch, precision=0)
+            log("omega (fp64): %.2f s" % (time.perf_counter() - t1))
             err = float(np.max(np.abs(r["score"] - r64["score"])))
         # FP64 oracle on the first regions, about cpu-seconds of one thread
         t0 = time.perf_counter()

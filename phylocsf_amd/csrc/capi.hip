@@ -166,7 +166,7 @@ int pbuild_failure(pcsf_model* m, unsigned long long key, const double* rho_host
 // branch, row) in the reference's order; its message is made on the host
 // from the same P(t) built in FP64 by Q.Diag.real_to_Pt's rules.
 int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int fmt,
-               float* ms_out) {
+               float* ms_out, unsigned outputs = kBuildP64 | kBuildMfma | kBuildTangent) {
   pcsf_ctx* c = m->ctx;
   HIP_TRY(c, hipMemsetAsync(m->d_perr, 0xff, sizeof(unsigned long long), c->stream));
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -176,7 +176,7 @@ int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int 
     HIP_TRY(c, hipEventRecord(e0, c->stream));
   }
   ps.fmt = -1;
-  HIP_TRY(c, launch_pbuild(m, ps, nsets, fmt, c->stream));
+  HIP_TRY(c, launch_pbuild(m, ps, nsets, fmt, outputs, c->stream));
   if (e1) HIP_TRY(c, hipEventRecord(e1, c->stream));
   unsigned long long key = ~0ull;
   HIP_TRY(c, hipMemcpyAsync(&key, m->d_perr, sizeof key, hipMemcpyDeviceToHost, c->stream));
@@ -189,7 +189,7 @@ int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int 
     (void)hipEventDestroy(e1);
   }
   if (key != ~0ull) return pbuild_failure(m, key, rho_host, ps.d_eVT != nullptr);
-  ps.fmt = fmt;
+  ps.fmt = (outputs & kBuildMfma) ? fmt : -1;
   return PCSF_OK;
 }
 
@@ -361,7 +361,8 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
       HIP_TRY(c, up(b->rs.d_elnpi, sm->lnpi, 64));
     }
     float tp = 0.f;
-    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, fmt, &tp);
+    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, fmt, &tp,
+                    fp64 ? kBuildP64 : kBuildMfma);
     if (rc) {
       char buf[64];
       std::snprintf(buf, sizeof buf, " (region block from %d)", r0);
@@ -394,7 +395,8 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
 }
 
 // Gradient pass at tree scales rho (nreg * nm) over the pairs in `active`
-// (optional): per-region lpr into d_gloglik and d lpr / d rho into d_grad
+// (optional): per-region lpr into d_gloglik, d lpr / d rho into d_grad and
+// the sum over columns of (d ln z / d rho)^2 into d_gsq
 // (the per-column buffers of the forward hold the column values).  tangent = false: the FP64
 // outside-pass adjoint, which also accumulates, if d_gbr is given, the
 // per-region sums of d ln z / d t_br (nreg * nm * nbr); tangent = true: the
@@ -437,7 +439,8 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
     HIP_TRY(c, hipMemcpyAsync(b->rs.d_active, act.data(), act.size(), hipMemcpyHostToDevice,
                               c->stream));
     float tp = 0.f;
-    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, kFragF32, &tp);
+    rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, kFragF32, &tp,
+                    tangent ? (kBuildMfma | kBuildTangent) : kBuildP64);
     if (rc) return rc;
     hipError_t e;
     if (tangent) {
@@ -449,7 +452,8 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
                               b->d_adj_scratch, adjoint_mfma_grid(m), c->stream);
     }
     HIP_TRY(c, e);
-    HIP_TRY(c, launch_region_reduce_aligned(b, ch, b->d_gloglik, b->d_grad, c->stream));
+    HIP_TRY(c, launch_region_reduce_aligned(b, ch, b->d_gloglik, b->d_grad, c->stream,
+                                            b->d_gsq));
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return PCSF_OK;
@@ -869,6 +873,7 @@ int pcsf_batch_create(pcsf_model* m, const uint8_t* codes, int64_t ncols,
   HIP_TRY(c, dalloc(&b->d_elpr, std::max<size_t>(1, (size_t)nreg * m->nm)));
   HIP_TRY(c, dalloc(&b->d_gloglik, std::max<size_t>(1, (size_t)nreg * m->nm)));
   HIP_TRY(c, dalloc(&b->d_grad, std::max<size_t>(1, (size_t)nreg * m->nm)));
+  HIP_TRY(c, dalloc(&b->d_gsq, std::max<size_t>(1, (size_t)nreg * m->nm)));
   HIP_TRY(c, hipMemcpy(b->d_offsets, off, sizeof(int64_t) * (nreg + 1), hipMemcpyHostToDevice));
   // upload through a staging buffer (with a flag word for invalid codes at
   // its end), then make leaf-major
@@ -937,6 +942,7 @@ int pcsf_batch_create_frames(pcsf_model* m, const uint8_t* nuc, int64_t n_nuc,
   HIP_TRY(c, dalloc(&b->d_elpr, std::max<size_t>(1, (size_t)nbr * m->nm)));
   HIP_TRY(c, dalloc(&b->d_gloglik, std::max<size_t>(1, (size_t)nbr * m->nm)));
   HIP_TRY(c, dalloc(&b->d_grad, std::max<size_t>(1, (size_t)nbr * m->nm)));
+  HIP_TRY(c, dalloc(&b->d_gsq, std::max<size_t>(1, (size_t)nbr * m->nm)));
   HIP_TRY(c, hipMemcpy(b->d_offsets, foff.data(), sizeof(int64_t) * foff.size(),
                        hipMemcpyHostToDevice));
   uint8_t* d_nuc = nullptr;
@@ -977,6 +983,7 @@ void pcsf_batch_destroy(pcsf_batch* b) {
   dfree(b->d_acolelpr);
   dfree(b->d_gloglik);
   dfree(b->d_grad);
+  dfree(b->d_gsq);
   dfree(b->d_adj_scratch);
   free_psets(b->rs);
   for (auto& ev : b->ev)
@@ -1484,25 +1491,35 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   // noisy at the 1e-3 level near the root (a few thousand nats of lpr), so
   // asking them for more than 1e-4 only buys bisection steps.
   const double tol = fp64_precision(precision) ? 1e-6 : 1e-4;
-  // Per pair: Newton steps on h = d lpr / d ln rho with h' from the last two
-  // evaluations (secant), inside the bracket [lo, hi] that the signs of h
-  // leave for the maximum; bisection when a step leaves the bracket or the
-  // bracket stops halving.
+  // A pair also stops when the Newton step from its last point promises a
+  // gain G = h^2 / (2 curv) that leaves nothing to measure, and takes that
+  // step.  With the secant's curvature within a factor 1 +- d of lpr's the
+  // step leaves ~d^2 G, d estimated by the change of the secant curvature
+  // between the last two steps (at least 0.1): FP64 asks d^2 G <=
+  // 1e-12 |lpr| (the oracle maximiser's digits), the MFMA precisions
+  // d^2 G <= 2e-5 nats (1e-4 dB; their final lpr is FP64's at the estimate).
+  const bool strict = fp64_precision(precision);
+  // Per pair: Newton steps on h = d lpr / d ln rho, inside the bracket
+  // [lo, hi] that the signs of h leave for the maximum, with h' from the last
+  // two evaluations (secant) where that is concave, else from the outer
+  // product of the column scores, sum_col (d ln z / d ln rho)^2 (BHHH: the
+  // expected information, whose first step lands near the maximum where a
+  // fixed first step would not); bisection when a step leaves the bracket or
+  // the bracket stops halving.
   struct G {
     bool done = false, have_prev = false, seen_pos = false, seen_neg = false;
     double x = 0, px = 0, ph = 0;
+    double pcurv = 0;  // the previous secant curvature (0: none yet)
     double lo = 0, hi = 0;
-    double w_check = 0;  // bracket width two steps ago
-    int since = 0;
+    int slow = 0;  // consecutive steps that did not halve |h|
   };
   std::vector<G> g(np);
   for (G& s : g) {
     s.lo = xlo;
     s.hi = xhi;
-    s.w_check = xhi - xlo;
   }
   st.assign(np, PairState());
-  std::vector<double> rho(np, 1.0), L(np), D(np);
+  std::vector<double> rho(np, 1.0), L(np), D(np), S(np);
   std::vector<uint8_t> act(np, 1);
   auto eval = [&]() -> int {
     // FP64: the outside-pass adjoint; MFMA precisions: the FP32 tangent pass
@@ -1510,6 +1527,7 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
     if (rc) return rc;
     HIP_TRY(c, hipMemcpy(L.data(), b->d_gloglik, np * 8, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(D.data(), b->d_grad, np * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(S.data(), b->d_gsq, np * 8, hipMemcpyDeviceToHost));
     return PCSF_OK;
   };
   for (int it = 0; it < 100; ++it) {
@@ -1558,18 +1576,48 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
         s.done = true;
         continue;
       }
-      double xn;
-      if (!s.have_prev) {
-        xn = s.x + (h > 0 ? 1.0 : -1.0);  // first move: a factor e in rho
-      } else {
+      double curv = 0.0;  // -h' in ln rho
+      bool secant = false;
+      if (s.have_prev) {
         const double hp = (h - s.ph) / (s.x - s.px);
-        xn = (hp < 0) ? s.x - h / hp : s.x + (h > 0 ? 2.0 : -2.0) * std::fabs(s.x - s.px);
+        if (hp < 0) {
+          curv = -hp;
+          secant = true;
+        }
+      }
+      // the outer product of the column scores about their mean, n times
+      // their variance: the mean's square, h^2 / n, would dominate the plain
+      // sum away from the maximum
+      const double n_cols = (double)(b->offsets[k / m->nm + 1] - b->offsets[k / m->nm]);
+      const double info = std::exp(2.0 * s.x) * S[k] - h * h / std::max(1.0, n_cols);
+      if (!secant && std::isfinite(info) && info > 0.0) curv = info;
+      double xn;
+      if (curv > 0.0) {
+        xn = s.x + h / curv;
+        if (secant) {
+          const double d = std::max(0.1, s.pcurv > 0.0 ? std::fabs(curv - s.pcurv) / curv : 1.0);
+          const double left = d * d * h * h / (2.0 * curv);  // what the step leaves
+          s.pcurv = curv;
+          if (left <= (strict ? 1e-12 * std::max(1.0, std::fabs(L[k])) : 2e-5)) {
+            if (xn > s.lo && xn < s.hi) s.x = xn;
+            s.done = true;
+            continue;
+          }
+        }
+        xn = std::min(std::max(xn, s.x - 2.0), s.x + 2.0);  // at most e^2 in rho
+      } else if (s.have_prev) {
+        xn = s.x + (h > 0 ? 2.0 : -2.0) * std::fabs(s.x - s.px);
+      } else {
+        xn = s.x + (h > 0 ? 1.0 : -1.0);  // a factor e in rho
       }
       const bool bracketed = s.seen_pos && s.seen_neg;
-      if (bracketed && ++s.since >= 2) {  // the bracket must halve every two steps
-        if (s.hi - s.lo > 0.5 * s.w_check) xn = 0.5 * (s.lo + s.hi);
-        s.w_check = s.hi - s.lo;
-        s.since = 0;
+      // Newton steps that converge from one side leave the bracket wide, so
+      // progress is judged by |h|: bisection after two steps in a row that
+      // did not halve it
+      s.slow = (s.have_prev && std::fabs(h) > 0.5 * std::fabs(s.ph)) ? s.slow + 1 : 0;
+      if (bracketed && s.slow >= 2) {
+        xn = 0.5 * (s.lo + s.hi);
+        s.slow = 0;
       }
       if (!(xn > s.lo && xn < s.hi))
         xn = bracketed ? 0.5 * (s.lo + s.hi) : std::min(std::max(xn, s.lo), s.hi);

@@ -155,6 +155,7 @@ struct pcsf_batch {
   // last forward's d_loglik / d_elpr stay as they were
   double* d_gloglik = nullptr;      // nreg * nm: lpr at the differentiated scales
   double* d_grad = nullptr;         // nreg * nm: d lpr / d rho
+  double* d_gsq = nullptr;          // nreg * nm: sum over columns of (d ln z / d rho)^2
   double* d_adj_scratch = nullptr;  // FP64 outside pass: node tiles per resident wave
   size_t adj_scratch = 0;           // its size in doubles
 };
@@ -172,8 +173,13 @@ struct RegionChunk {
 // (0..63) nor PCSF_MARGINALIZE
 hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uint8_t* dst,
                                   int64_t ld, int* bad, hipStream_t s);
+// What a transition-set build writes (the sets hold room for it): FP64 P^T
+// (P64T, if the sets have it), the MFMA kernels' leaf rows and tiles in
+// format fmt (PT32, Pfrag, lnPcc), the tangent's D (DT32, Dfrag, if the sets
+// have them).
+constexpr unsigned kBuildP64 = 1, kBuildMfma = 2, kBuildTangent = 4;
 hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int fmt,
-                         hipStream_t s);
+                         unsigned outputs, hipStream_t s);
 hipError_t launch_cherry_tables(const pcsf_model* m, const PSets& sets, int nsets,
                                 hipStream_t s);
 hipError_t launch_prune_fp64(const pcsf_model* m, const pcsf_batch* b, hipStream_t s);
@@ -222,8 +228,11 @@ hipError_t launch_forward_fp64m(const pcsf_model* m, const pcsf_batch* b, const 
 hipError_t launch_prune_tangent_regions(const pcsf_model* m, const pcsf_batch* b,
                                         const RegionChunk& ch, hipStream_t s);
 // sums over each region of the chunk into loglik / elpr (nreg * nm)
+// (sq, optional: the sums of the squared per-column second values -- with
+// d ln z / d rho there, the outer-product estimate of lpr's curvature)
 hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& ch,
-                                        double* loglik, double* elpr, hipStream_t s);
+                                        double* loglik, double* elpr, hipStream_t s,
+                                        double* sq = nullptr);
 // Launch shapes of the fused kernel that the autotuner chooses among.
 struct MfmaVariant {
   int wpg;        // waves per workgroup (16 columns each) sharing one LDS P tile

@@ -51,6 +51,27 @@ constexpr int kTileDoubles = 64 * 16;        // one 64 x 16 tile
 // ((row / 16) * 16 + col / 4) * 64 + (row % 16) + 16 (col % 4).  `src`
 // gives M either as M[row][col] = src[row * 64 + col] (by_rows) or as
 // M[row][col] = src[col * 64 + row]; the global reads run in src order.
+// fetch() takes a thread's share (elements tid + 256 k) into registers, so
+// that the next tile's reads are in flight during a matrix product; put()
+// writes the share to LDS.
+constexpr int kShare = 4096 / (64 * kWaves);
+struct Share {
+  double v[kShare];
+};
+__device__ __forceinline__ void fetch(Share& r, const double* __restrict__ src, int tid) {
+#pragma unroll
+  for (int k = 0; k < kShare; ++k) r.v[k] = src[tid + 64 * kWaves * k];
+}
+__device__ __forceinline__ void put(double* __restrict__ frag, const Share& r, bool by_rows,
+                                    int tid) {
+#pragma unroll
+  for (int k = 0; k < kShare; ++k) {
+    const int e = tid + 64 * kWaves * k, hi = e >> 6, lo = e & 63;
+    const int row = by_rows ? hi : lo, col = by_rows ? lo : hi;
+    frag[((row >> 4) * 16 + (col >> 2)) * 64 + (row & 15) + 16 * (col & 3)] = r.v[k];
+  }
+}
+// both at once, a few elements at a time (fewer registers)
 __device__ __forceinline__ void stage(double* __restrict__ frag, const double* __restrict__ src,
                                       bool by_rows, int tid) {
 #pragma unroll 4
@@ -123,7 +144,9 @@ __device__ __forceinline__ double column_dot(const Tile& x, const Tile& y) {
 template <bool kFwd>
 __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArgs p) {
   __shared__ double Qf[kFwd ? 1 : 4096];  // Q of the task's model, A-fragment order
-  __shared__ double Af[4096];             // the current P_c or P_c^T
+  // the current P_c or P_c^T; the forward alternates two buffers, so that one
+  // barrier per node suffices
+  __shared__ double Af[kFwd ? 2 * 4096 : 4096];
   extern __shared__ double gsum[];        // nbr: per-branch sums of the task's workgroup
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, grp = lane >> 4;
   const int j = lane & 15;
@@ -152,8 +175,14 @@ __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArg
       for (int br = tid; br < nbr; br += 64 * kWaves) gsum[br] = 0.0;
       __syncthreads();  // Qf and gsum ready (a two-leaf tree has no barrier before its use)
     }
-    // ---- inside pass (PhyloLik.ml:78-92), messages to the slots
+    // ---- inside pass (PhyloLik.ml:78-92), messages to the slots; nodes
+    // nl .. n - 2 take a product with their P, whose tile the forward
+    // fetches one node ahead (the outside pass's registers leave the
+    // adjoint no room for it)
     double z = 0.0;
+    Share nxt;
+    if (kFwd && nl <= n - 2) fetch(nxt, PT + (size_t)nl * 4096, tid);
+    int buf = 0;
     for (int i = nl; i < n; ++i) {
       const int l = p.lc[i], rr = p.rc[i];
       Tile a = (l < nl) ? leaf_message(PT + (size_t)l * 4096, p.codes[(size_t)l * p.ld + col], grp)
@@ -187,10 +216,20 @@ __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArg
         }
         break;
       }
-      __syncthreads();  // Af free
-      stage(Af, PT + (size_t)i * 4096, false, tid);
-      __syncthreads();
-      store(slots + (size_t)(i - nl) * kTileDoubles, gemm(Af, a, lane), lane);
+      double* A = Af + (kFwd ? buf * 4096 : 0);
+      if (kFwd) {
+        // buffer `buf` was last read two nodes ago, before the previous
+        // node's barrier
+        put(A, nxt, false, tid);
+        __syncthreads();
+        if (i + 1 <= n - 2) fetch(nxt, PT + (size_t)(i + 1) * 4096, tid);
+      } else {
+        __syncthreads();  // Af free
+        stage(A, PT + (size_t)i * 4096, false, tid);
+        __syncthreads();
+      }
+      store(slots + (size_t)(i - nl) * kTileDoubles, gemm(A, a, lane), lane);
+      buf ^= 1;
     }
     if (kFwd) continue;
     const double zinv = (z != 0.0) ? 1.0 / z : 0.0;

@@ -94,29 +94,34 @@ __global__ __launch_bounds__(256) void region_reduce_aligned_kernel(
     const double* __restrict__ logz, const double* __restrict__ colelpr,
     const int64_t* __restrict__ off, const int64_t* __restrict__ astart,
     const uint8_t* __restrict__ active, int r0, int nm, int64_t ald,
-    double* __restrict__ loglik, double* __restrict__ elpr) {
-  __shared__ double s0[256], s1[256];
+    double* __restrict__ loglik, double* __restrict__ elpr, double* __restrict__ sq) {
+  __shared__ double s0[256], s1[256], s2[256];
   const int r = r0 + blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
   if (active && !active[(size_t)blockIdx.x * nm + q]) return;
   const int64_t lo = astart[r], hi = lo + (off[r + 1] - off[r]);
-  double a = 0.0, b = 0.0;
+  double a = 0.0, b = 0.0, d = 0.0;
   for (int64_t c = lo + tid; c < hi; c += 256) {
     a += logz[(size_t)q * ald + c];
-    b += colelpr[(size_t)q * ald + c];
+    const double e = colelpr[(size_t)q * ald + c];
+    b += e;
+    d = fma(e, e, d);
   }
   s0[tid] = a;
   s1[tid] = b;
+  s2[tid] = d;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (tid < w) {
       s0[tid] += s0[tid + w];
       s1[tid] += s1[tid + w];
+      s2[tid] += s2[tid + w];
     }
     __syncthreads();
   }
   if (tid == 0) {
     loglik[(size_t)r * nm + q] = s0[0];
     elpr[(size_t)r * nm + q] = s1[0];
+    if (sq) sq[(size_t)r * nm + q] = s2[0];
   }
 }
 
@@ -132,14 +137,15 @@ hipError_t launch_align_codes(const pcsf_batch* b, hipStream_t s) {
 }
 
 hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& ch,
-                                        double* loglik, double* elpr, hipStream_t s) {
+                                        double* loglik, double* elpr, hipStream_t s,
+                                        double* sq) {
   if (ch.nreg == 0) return hipSuccess;
   dim3 grid(ch.nreg, b->m->nm);
   // the model's own sets (one per model, shared by every region) are all live
   const uint8_t* active = (ch.sets == &b->m->fixed) ? nullptr : ch.sets->d_active;
   hipLaunchKernelGGL(region_reduce_aligned_kernel, grid, dim3(256), 0, s, b->d_alogz,
                      b->d_acolelpr, b->d_offsets, b->al.d_astart, active, ch.r0,
-                     b->m->nm, b->al.ld, loglik, elpr);
+                     b->m->nm, b->al.ld, loglik, elpr, sq);
   return hipGetLastError();
 }
 

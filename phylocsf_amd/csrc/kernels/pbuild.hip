@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     // A leaf branch is read only through its leaf messages, an internal branch
     // only through its tile: each writes just its own kind.
     const bool leaf = br < nl;
-    if (leaf) {
+    if (leaf && PT32) {
       float* out = PT32 + sb * (kLeafRows * 64);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
         lnPcc[sb * kLeafRows + tid] = log(pdiag[tid]);
       }
       if (tid == 64) lnPcc[sb * kLeafRows + 64] = 0.0;
-    } else {
+    } else if (!leaf && Pfrag) {
       // The tile: the off-diagonal of P (diagonal entries 0), so the products
       // sum nonnegative terms, and the diagonal P[a][a] - 1 joins in
       // round-to-nearest VALU arithmetic.  FP32: A-operand order of
@@ -375,13 +375,16 @@ hipError_t launch_cherry_tables(const pcsf_model* m, const PSets& sets, int nset
 }
 
 hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int fmt,
-                         hipStream_t s) {
+                         unsigned outputs, hipStream_t s) {
   if (nsets <= 0) return hipSuccess;
+  const bool p64 = (outputs & kBuildP64) && sets.fp64;
+  const bool mfma = (outputs & kBuildMfma) != 0;
+  const bool tangent = (outputs & kBuildTangent) && sets.d_Dfrag;
   // several branches per workgroup once there are enough matrices to fill
   // the device; one each for the few sets of a fixed-strategy model
   const int nbw = ((size_t)nsets * m->nbr >= 16384) ? 8 : 1;
   dim3 grid((m->nbr + nbw - 1) / nbw, nsets);
-  auto* k = sets.d_Dfrag ? pbuild_kernel<true> : pbuild_kernel<false>;
+  auto* k = tangent ? pbuild_kernel<true> : pbuild_kernel<false>;
   // the eigensystem of set s is model s % nm's, or the set's own when the
   // sets carry rate models (then s % cap = s)
   const bool own = sets.d_eVT != nullptr;
@@ -389,8 +392,8 @@ hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int 
                      own ? sets.d_eVinv : m->d_Vinv, own ? sets.d_elam : m->d_lam,
                      m->d_branch, sets.d_rho, sets.d_active, own ? sets.cap : m->nm, m->nbr,
                      nbw, m->nl,
-                     sets.fp64 ? sets.d_P64T : nullptr, sets.d_PT32, sets.d_Pfrag, fmt,
-                     sets.d_lnPcc, m->d_perr, sets.d_DT32,
+                     p64 ? sets.d_P64T : nullptr, mfma ? sets.d_PT32 : nullptr,
+                     mfma ? sets.d_Pfrag : nullptr, fmt, sets.d_lnPcc, m->d_perr, sets.d_DT32,
                      sets.d_Dfrag);
   return hipGetLastError();
 }
