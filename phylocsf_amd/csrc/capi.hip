@@ -1504,11 +1504,14 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   // two evaluations (secant) where that is concave, else from the outer
   // product of the column scores, sum_col (d ln z / d ln rho)^2 (BHHH: the
   // expected information, whose first step lands near the maximum where a
-  // fixed first step would not); bisection when a step leaves the bracket or
-  // the bracket stops halving.
+  // fixed first step would not).  While the last two points are more than
+  // 0.01 apart the step goes to the maximum of the cubic through their lpr
+  // and h instead (Hermite interpolation: one order above the secant, and
+  // the lpr differences still far above their rounding there).  Bisection
+  // when a step leaves the bracket or two steps in a row fail to halve |h|.
   struct G {
     bool done = false, have_prev = false, seen_pos = false, seen_neg = false;
-    double x = 0, px = 0, ph = 0;
+    double x = 0, px = 0, ph = 0, pL = 0;
     double pcurv = 0;  // the previous secant curvature (0: none yet)
     double lo = 0, hi = 0;
     int slow = 0;  // consecutive steps that did not halve |h|
@@ -1604,6 +1607,20 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
             continue;
           }
         }
+        if (s.have_prev && std::fabs(s.x - s.px) > 1e-2) {
+          // maximum of the cubic with lpr and h at px and x: Nocedal and
+          // Wright's interpolation step (Numerical Optimization, eq. 3.59)
+          // on phi = -lpr
+          const double f0 = -s.pL, g0 = -s.ph, f1 = -L[k], g1 = -h;
+          const double d1 = g0 + g1 - 3.0 * (f0 - f1) / (s.px - s.x);
+          const double disc = d1 * d1 - g0 * g1;
+          if (disc >= 0.0) {
+            const double d2 = std::copysign(std::sqrt(disc), s.x - s.px);
+            const double den = g1 - g0 + 2.0 * d2;
+            const double xc = s.x - (s.x - s.px) * (g1 + d2 - d1) / den;
+            if (den != 0.0 && std::isfinite(xc)) xn = xc;
+          }
+        }
         xn = std::min(std::max(xn, s.x - 2.0), s.x + 2.0);  // at most e^2 in rho
       } else if (s.have_prev) {
         xn = s.x + (h > 0 ? 2.0 : -2.0) * std::fabs(s.x - s.px);
@@ -1628,6 +1645,7 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
       }
       s.px = s.x;
       s.ph = h;
+      s.pL = L[k];
       s.have_prev = true;
       s.x = xn;
     }
