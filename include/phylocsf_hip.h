@@ -214,7 +214,7 @@ int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
  *                        model, all regions advancing in lock step on the GPU
  * At an MFMA precision (PCSF_FP32 .. PCSF_BF16) the maximisers search on
  * that precision's likelihoods and take lpr and elpr at their estimates from
- * one FP64 forward (the MFMA rounding, up to ~6e-9 of lpr on regions of
+ * one FP64 forward (PCSF_FP64M) (the MFMA rounding, up to ~6e-9 of lpr on regions of
  * thousands of codons at per-region scales, would otherwise reach the score).
  * score / anc_comp in decibans (n_regions); rho, lpr, elpr per region and
  * model (n_regions*2, coding first); status per region (PCSF_OK, or
@@ -223,12 +223,16 @@ int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
 #define PCSF_STRATEGY_FIXED 0
 #define PCSF_STRATEGY_MLE 1
 /* mle with the gradient: root of h = d lpr / d ln rho on [0.01, 10] by
- * Newton steps with h' from the last two evaluations, kept inside the
- * bracket the signs of h leave and bisected when it stops halving, to 1e-6
- * in ln rho (1e-4 with FP32 gradients); the boundary when h keeps its sign
- * there.  Finds the maximum
- * the reference's 1%-tolerance Brent approximates.  The gradient is the FP64
- * outside pass at PCSF_FP64, the FP32 tangent pass otherwise. */
+ * Newton steps inside the bracket the signs of h leave -- the first with the
+ * column scores' centred outer product as the curvature, then to the
+ * maximum of the cubic through the last two points' lpr and h while they
+ * are more than 0.01 apart, the secant's Newton point when closer --
+ * bisecting when a step leaves the bracket or two steps fail to halve |h|;
+ * a pair stops when its next step would leave less than 1e-12 |lpr| (FP64,
+ * FP64M) or 2e-5 nats (the MFMA precisions), at 1e-6 (1e-4) in ln rho, or at
+ * a bound where h keeps its sign.  Finds the maximum the reference's
+ * 1%-tolerance Brent approximates.  The gradient is the FP64 outside pass at
+ * PCSF_FP64 and PCSF_FP64M, the FP32 tangent pass otherwise. */
 #define PCSF_STRATEGY_MLE_GRAD 2
 int pcsf_score(pcsf_batch *b, int strategy, int precision, double *score, double *anc_comp,
                double *rho, double *lpr, double *elpr, int32_t *status, int32_t *n_evals);

@@ -12,10 +12,10 @@
 // still written alignment by alignment, in input order.
 //
 // Extra options (no counterpart in the reference):
-//   --precision=fp64|fp64m|fp32|fp16s   arithmetic of the pruning (default
-//                                 fp64: the reference's digits; fp64m the same
-//                                 FP64 arithmetic on matrix cores; fp32 and
-//                                 split-fp16 agree with it to 1e-3 dB per
+//   --precision=fp64m|fp64|fp32|fp16s   arithmetic of the pruning (default
+//                                 fp64m: FP64 on matrix cores, the reference's
+//                                 digits; fp64 the plain FP64 kernel; fp32 and
+//                                 split-fp16 agree with them to 1e-3 dB per
 //                                 region at fixed scales)
 //   --device=N                    GPU index (default 0)
 //   --batchBytes=N                leaf-code bytes packed per GPU call (default 1 GiB)
@@ -96,7 +96,7 @@ struct Options {
   bool all_scores = false;
   int procs = 1;
   bool bls = false, anc_comp = false, dna = false, aa = false, debug = false;
-  int precision = PCSF_FP64;
+  int precision = PCSF_FP64M;
   int device = 0;
   long long batch_bytes = 1LL << 30;
   std::vector<std::string> positional;
@@ -129,7 +129,7 @@ const char* kUsage =
     "    --aa                       include amino acid translation in output\n"
     "    --debug                    print extra information about parameters and errors\n\n"
     "  GPU:\n"
-    "    --precision=fp64|fp64m|fp32|fp16s  arithmetic of the pruning kernel (default fp64)\n"
+    "    --precision=fp64m|fp64|fp32|fp16s  arithmetic of the pruning kernel (default fp64m)\n"
     "    --device=N                 GPU index (default 0)\n"
     "    --batchBytes=N             leaf-code bytes packed per GPU call (default 1 GiB)\n";
 

@@ -2,9 +2,10 @@
 
 * the reference's four published CLI results (src/test.ml:27-59: tal-AA,
   ALDH2 exon 5 out of frame and in 6 frames, Aldh2 mRNA ATGStop with the
-  translation), with the same command lines, at FP32 (default) and FP64;
-* the fixed strategy against the oracle's restatement of the CLI
-  (oracle/cli_oracle.py) on every frame of the examples;
+  translation), with the same command lines, at FP32, FP64 (the plain
+  kernel) and FP64M (FP64 on matrix cores, the default);
+* the fixed strategy (default precision) against the oracle's restatement of
+  the CLI (oracle/cli_oracle.py) on every frame of the examples;
 * several alignments in one run (--files) give the same lines as one run each.
 """
 import os
@@ -33,7 +34,7 @@ def run(*args):
     return [l.split("\t") for l in p.stdout.strip().split("\n")]
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp64m"])
 def test_published_cli_results(examples_dir, precision):
     ex = lambda f: os.path.join(examples_dir, f)  # noqa: E731
     pr = "--precision=" + precision
