@@ -273,14 +273,11 @@ size_t set_bytes(const pcsf_model* m, bool fp64, bool tangent = false, bool own_
          (own_model ? (2 * 4096 + 3 * 64) * 8 : 0);
 }
 
-// Rate models of their own for every (region, model) pair k = r * nm + q
-// (host arrays): V transposed, V^-1, eigenvalues, root prior and its log.
+// Rate models of their own for every (region, model) pair k = r * nm + q:
+// the omega model's twelve settings per pair (host array, np * 12), from
+// which each chunk's eigensystems and root priors are built on the GPU.
 struct SetModels {
-  const double* VT;     // np * 4096
-  const double* Vinv;   // np * 4096
-  const double* lam;    // np * 64
-  const double* pi;     // np * 64
-  const double* lnpi;   // np * 64
+  const double* qs;
 };
 
 // Node tiles of the FP64 matrix-core kernel (forward and outside pass): one
@@ -348,17 +345,17 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
                               hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(b->rs.d_active, act.data(), act.size(), hipMemcpyHostToDevice,
                               c->stream));
-    if (own) {  // the chunk's rate models
+    if (own) {  // the chunk's rate models, built from their settings
       const size_t k0 = (size_t)r0 * nm, nk = (size_t)nr * nm;
-      auto up = [&](double* dst, const double* src, size_t per) {
-        return hipMemcpyAsync(dst, src + k0 * per, nk * per * 8, hipMemcpyHostToDevice,
-                              c->stream);
-      };
-      HIP_TRY(c, up(b->rs.d_eVT, sm->VT, 4096));
-      HIP_TRY(c, up(b->rs.d_eVinv, sm->Vinv, 4096));
-      HIP_TRY(c, up(b->rs.d_elam, sm->lam, 64));
-      HIP_TRY(c, up(b->rs.d_epi, sm->pi, 64));
-      HIP_TRY(c, up(b->rs.d_elnpi, sm->lnpi, 64));
+      if (b->setqs_cap < (size_t)b->rs.cap * 12) {
+        dfree(b->d_setqs);
+        b->setqs_cap = 0;
+        HIP_TRY(c, dalloc(&b->d_setqs, (size_t)b->rs.cap * 12));
+        b->setqs_cap = (size_t)b->rs.cap * 12;
+      }
+      HIP_TRY(c, hipMemcpyAsync(b->d_setqs, sm->qs + k0 * 12, nk * 12 * 8, hipMemcpyHostToDevice,
+                                c->stream));
+      HIP_TRY(c, launch_omega_eigen(b->d_setqs, (int)nk, b->rs, c->stream));
     }
     float tp = 0.f;
     rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, fmt, &tp,
@@ -985,6 +982,7 @@ void pcsf_batch_destroy(pcsf_batch* b) {
   dfree(b->d_grad);
   dfree(b->d_gsq);
   dfree(b->d_adj_scratch);
+  dfree(b->d_setqs);
   free_psets(b->rs);
   for (auto& ev : b->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -1686,7 +1684,6 @@ struct OmegaPair {
   double lpr = 0;
   int status = PCSF_OK;
   int evals = 0;
-  double eig_kappa = -1;     // kappa of the eigensystem in the host arrays
 };
 
 int score_omega(pcsf_batch* b, int precision, double omega_h1, double sigma_h1,
@@ -1715,72 +1712,40 @@ int score_omega(pcsf_batch* b, int precision, double omega_h1, double sigma_h1,
       if (h == 0) p.mx.start(p.rho, 0.001, 10.0, 0.01, rel_tol);
     }
   }
-  std::vector<double> VT(np * 4096), Vi(np * 4096), lam(np * 64), pri(np * 64), lnpri(np * 64);
+  std::vector<double> qs_all(np * pcsf::kOmegaSettings);
   std::vector<double> rho(np, 1.0), L(np);
   std::vector<uint8_t> act(np);
-  SetModels sm{VT.data(), Vi.data(), lam.data(), pri.data(), lnpri.data()};
-  // the rate model of pair k at kappa x: eigensystem and root prior
-  auto build = [&](size_t k, double x) -> bool {
-    OmegaPair& p = st[k];
-    double qs[pcsf::kOmegaSettings];
-    std::copy(p.qs, p.qs + pcsf::kOmegaSettings, qs);
+  SetModels sm{qs_all.data()};
+  // the settings of pair k's rate model at kappa x (the GPU builds Q, its
+  // eigensystem and the root prior from them); false where the reference's
+  // rate-matrix construction would fail: a codon frequency or the mean rate
+  // not positive
+  auto settings = [&](size_t k, double x) -> bool {
+    double* qs = qs_all.data() + k * pcsf::kOmegaSettings;
+    std::copy(st[k].qs, st[k].qs + pcsf::kOmegaSettings, qs);
     qs[0] = x;
-    std::vector<double> S(4096), Q(4096);
     double pi[64];
-    pcsf::omega_exchangeabilities(qs, S.data());
     pcsf::omega_pi(qs, pi);
-    pcsf::Eigen e;
-    if (!pcsf::build_rate_matrix(64, S.data(), pi, Q.data(), nullptr) ||
-        !pcsf::eigen_reversible(64, Q.data(), pi, &e, nullptr))
-      return false;
-    // Q.Diag.equilibrium: the stationary distribution, pi normalised
-    double tot = 0.0;
-    for (double v : pi) tot += v;
-    for (int i = 0; i < 64; ++i) {
-      for (int j = 0; j < 64; ++j) VT[k * 4096 + j * 64 + i] = e.V[i * 64 + j];
-      pri[k * 64 + i] = pi[i] / tot;
-      lnpri[k * 64 + i] = std::log(pi[i] / tot);
-    }
-    std::copy(e.Vinv.begin(), e.Vinv.end(), Vi.begin() + k * 4096);
-    std::copy(e.lam.begin(), e.lam.end(), lam.begin() + k * 64);
-    p.eig_kappa = x;
-    return true;
+    for (double v : pi)
+      if (!(v > 0.0) || !std::isfinite(v)) return false;
+    return x > 0.0 && qs[1] > 0.0;
   };
-  const int nthreads =
-      (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
   for (int step = 0; step < 1000000; ++step) {
     bool any = false;
-    std::vector<size_t> need;
     for (size_t k = 0; k < np; ++k) {
       OmegaPair& p = st[k];
       act[k] = p.phase < 2 && !p.waiting && p.status == PCSF_OK;
       if (!act[k]) continue;
-      any = true;
       const double x = p.mx.pending();
       rho[k] = p.phase == 0 ? x : p.rho;
-      const double kap = p.phase == 1 ? x : p.kappa;
-      if (kap != p.eig_kappa) need.push_back(k);
+      if (!settings(k, p.phase == 1 ? x : p.kappa)) {
+        p.status = PCSF_EFAIL;
+        act[k] = 0;
+        continue;
+      }
+      any = true;
     }
     if (!any) break;
-    // new eigensystems on host threads
-    {
-      std::vector<std::thread> pool;
-      std::vector<char> ok(need.size(), 1);
-      for (int t = 0; t < nthreads; ++t)
-        pool.emplace_back([&, t]() {
-          for (size_t i = t; i < need.size(); i += nthreads) {
-            const size_t k = need[i];
-            const double kap = st[k].phase == 1 ? st[k].mx.pending() : st[k].kappa;
-            ok[i] = build(k, kap) ? 1 : 0;
-          }
-        });
-      for (auto& th : pool) th.join();
-      for (size_t i = 0; i < need.size(); ++i)
-        if (!ok[i]) {
-          st[need[i]].status = PCSF_EFAIL;
-          act[need[i]] = 0;
-        }
-    }
     rc = forward_regions(b, rho.data(), act.data(), precision, &sm);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));
@@ -1832,17 +1797,14 @@ int score_omega(pcsf_batch* b, int precision, double omega_h1, double sigma_h1,
   if (!fp64_precision(precision)) {
     // each hypothesis's lpr at its estimates in FP64 (the fits ran on the
     // MFMA precision's likelihoods): the last kappa step's f = lpr_kappa + lpr
-    std::vector<size_t> need;
     for (size_t k = 0; k < np; ++k) {
       act[k] = st[k].status == PCSF_OK;
       rho[k] = st[k].rho;
-      if (act[k] && st[k].kappa != st[k].eig_kappa) need.push_back(k);
-    }
-    for (size_t k : need)
-      if (!build(k, st[k].kappa)) {
+      if (act[k] && !settings(k, st[k].kappa)) {
         st[k].status = PCSF_EFAIL;
         act[k] = 0;
       }
+    }
     rc = forward_regions(b, rho.data(), act.data(), PCSF_FP64M, &sm);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpy(L.data(), b->d_loglik, np * 8, hipMemcpyDeviceToHost));

@@ -158,6 +158,8 @@ struct pcsf_batch {
   double* d_gsq = nullptr;          // nreg * nm: sum over columns of (d ln z / d rho)^2
   double* d_adj_scratch = nullptr;  // FP64 outside pass: node tiles per resident wave
   size_t adj_scratch = 0;           // its size in doubles
+  double* d_setqs = nullptr;        // per-set model settings of a chunk (omega test)
+  size_t setqs_cap = 0;             // its size in doubles
 };
 
 // Per-region-scale launch description (one chunk of regions).
@@ -225,6 +227,10 @@ hipError_t launch_adjoint_mfma(const pcsf_model* m, const pcsf_batch* b, const R
 // column into the batch's aligned per-column buffers (precision PCSF_FP64M)
 hipError_t launch_forward_fp64m(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
                                 double* scratch, int grid, hipStream_t s);
+// the omega test's rate model of every active set of a chunk from its twelve
+// settings (d_qs: nsets * 12): eigensystem and root prior into the sets'
+// own-model arrays
+hipError_t launch_omega_eigen(const double* d_qs, int nsets, const PSets& sets, hipStream_t s);
 hipError_t launch_prune_tangent_regions(const pcsf_model* m, const pcsf_batch* b,
                                         const RegionChunk& ch, hipStream_t s);
 // sums over each region of the chunk into loglik / elpr (nreg * nm)

@@ -3,11 +3,16 @@ restatement of OmegaModel.score (src/OmegaModel.ml:160-219; paths relative to
 the mlin/PhyloCSF tree).
 
 With FP64 likelihoods the library's lock-step kr_map follows the oracle's
-maximiser (find_init, GSL Brent, three rounds of coordinate ascent), so the
-scores agree to 1e-5 dB and the estimates to 1e-5 (Brent's last steps are
-sqrt(eps)-sized, and on a flat optimum a comparison there can fall either way
-on 1e-12-level differences of the two eigensolvers; six chained maximisations
-carry such a difference along), on simulated regions and
+maximiser (find_init, GSL Brent, three rounds of coordinate ascent): each
+Brent run stops once its bracket is within 1% of its estimate, and on the
+flat optimum of kappa (likelihood and gamma prior) a comparison in the last
+steps can fall either way on the 1e-13-level differences between the
+library's GPU eigensolver (parallel-order Jacobi) and the oracle's; six
+chained maximisations carry such a difference along.  The estimates then
+agree to 1e-4 relative (measured: 1.3e-5 at most), each hypothesis's MAP lpr
+-- flat to second order there -- to 1e-8 relative (measured: 1.4e-9), and the
+scores to what that allows, 1e-5 dB + DB 1e-8 (|lpr_H0| + |lpr_H1|).
+Checked on simulated regions and
 on the reference's example alignments, for the default H1 (omega 0.2, sigma
 0.01) and another one.  The MFMA precisions change the likelihoods by their
 rounding only; the maximiser may then stop elsewhere inside its 1% intervals,
@@ -44,15 +49,21 @@ def _sim_regions(paramset, lengths, seed):
                       seed=seed)
 
 
-def _check(res, codes, off, otree, h1, tol_rho=1e-5, tol_db=1e-5):
+def _check(res, codes, off, otree, h1, tol_rho=1e-4, tol_db=1e-5):
+    worst = [0.0, 0.0]
     for r in range(len(off) - 1):
         ref = O.omega_score(otree, codes[off[r]:off[r + 1]], *h1)
+        worst[0] = max(worst[0], abs(res["score"][r] - ref["score"]))
+        worst[1] = max(worst[1], max(abs(res["kappa"][r, h] / ref["kappa_" + k] - 1)
+                                     for h, k in ((0, "H0"), (1, "H1"))))
         assert res["status"][r] == 0, r
-        assert abs(res["score"][r] - ref["score"]) <= tol_db, (r, res["score"][r], ref["score"])
+        tol = tol_db + P.DB * 1e-8 * (abs(ref["lpr_H0"]) + abs(ref["lpr_H1"]))
+        assert abs(res["score"][r] - ref["score"]) <= tol, (r, res["score"][r], ref["score"])
         for h, key in ((0, "H0"), (1, "H1")):
             assert abs(res["rho"][r, h] - ref["rho_" + key]) <= tol_rho * ref["rho_" + key], (r, h)
             assert abs(res["kappa"][r, h] - ref["kappa_" + key]) <= tol_rho * ref["kappa_" + key]
-            assert abs(res["lpr"][r, h] - ref["lpr_" + key]) <= 1e-9 * abs(ref["lpr_" + key])
+            assert abs(res["lpr"][r, h] - ref["lpr_" + key]) <= 1e-8 * abs(ref["lpr_" + key])
+    _say("  max |d score| %.3g dB, max relative d kappa %.3g" % tuple(worst))
 
 
 @pytest.mark.parametrize("paramset", ["12flies", "29mammals"])
