@@ -452,8 +452,9 @@ class PhyloLik:
     def __init__(self, ctx: Context):
         self.ctx = ctx
 
-    def prepare(self, tree: Tree, pms: np.ndarray, prior: np.ndarray, leaves: np.ndarray,
-                leaf_dist: Optional[np.ndarray] = None):
+    @staticmethod
+    def _inputs(tree, pms, prior, leaves, leaf_dist):
+        """The checks of PhyloLik.prepare; -> (P, prior, leaves, leaf_dist, children)."""
         k = len(prior)
         leaves = np.ascontiguousarray(leaves, dtype=np.int32)
         if leaves.ndim == 1:
@@ -469,10 +470,15 @@ class PhyloLik:
         ld = None if leaf_dist is None else np.ascontiguousarray(leaf_dist, dtype=np.float64)
         if ld is not None and ld.shape != (leaves.shape[0], tree.nl, k):
             raise InvalidArgument(-1, "CamlPaml.Infer.prepare: leaf_dist must be [ncols, nl, k]")
+        return P, pr, leaves, ld, np.ascontiguousarray(tree.children, dtype=np.int32)
+
+    def prepare(self, tree: Tree, pms: np.ndarray, prior: np.ndarray, leaves: np.ndarray,
+                leaf_dist: Optional[np.ndarray] = None):
+        k = len(prior)
+        P, pr, leaves, ld, ch = self._inputs(tree, pms, prior, leaves, leaf_dist)
         ncols = leaves.shape[0]
         z = np.zeros(ncols)
         post = np.zeros((ncols, k))
-        ch = np.ascontiguousarray(tree.children, dtype=np.int32)
         if ld is None:
             self.ctx.check(lib().pcsf_lik_columns(self.ctx.h, k, tree.n, _i32(ch), _dp(P),
                                                   _dp(pr), _i32(leaves), ncols, _dp(z),
@@ -552,7 +558,18 @@ def collect_sufficient_statistics(ctx: Context, tree: Tree, pms, prior, leaves, 
                                   leaf_dist=None, weights=None):
     """PhyloEM.collect_sufficient_statistics (lib/CamlPaml/PhyloEM.ml:14-20) over
     many columns: adds every column's branch posteriors (times its weight) to
-    ss [n-1, k, k] and returns the likelihood of each column."""
-    it = PhyloLik(ctx).prepare(tree, pms, prior, leaves, leaf_dist)
-    it.add_branch_posteriors(ss, 1.0 if weights is None else weights)
-    return it.likelihood()
+    ss [n-1, k, k] and returns the likelihood of each column.  One call of
+    pcsf_lik_posteriors (with 64 states and no `Distribution leaves: the FP64
+    matrix-core E-step)."""
+    P, pr, lv, ld, ch = PhyloLik._inputs(tree, pms, prior, leaves, leaf_dist)
+    k = len(pr)
+    if ss.shape != (tree.n - 1, k, k) or ss.dtype != np.float64 or not ss.flags.c_contiguous:
+        raise InvalidArgument(-1, "collect_sufficient_statistics: ss must be a C-contiguous "
+                                  "float64 [n-1, k, k] array")
+    z = np.zeros(lv.shape[0])
+    w = None if weights is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(weights, np.float64), z.shape))
+    ctx.check(lib().pcsf_lik_posteriors(ctx.h, k, tree.n, _i32(ch), _dp(P), _dp(pr), _i32(lv),
+                                        None if ld is None else _dp(ld), len(z),
+                                        None if w is None else _dp(w), _dp(z), None, _dp(ss)))
+    return z

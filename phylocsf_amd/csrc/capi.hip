@@ -1968,6 +1968,93 @@ namespace {
 // (lib/CamlPaml/PhyloLik.ml:46-174) over n_cols columns: the host side of
 // kernels/lik.hip.  Columns go in chunks whose device buffers stay within
 // ~512 MB; post_mode 1 = the root's posterior, 2 = every node's.
+// The E-step for 64 states without `Distribution leaves on the FP64 matrix
+// cores (kernels/estep.hip): columns in chunks of up to 32,768, u and v of
+// every (branch, column) of a chunk in device memory, the counts added per
+// chunk.
+int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
+               const std::vector<int32_t>& rc, const double* P, const double* prior,
+               const int32_t* leaves, int64_t ncols, const double* weights, double* z,
+               double* ecounts) {
+  const int nl = (n + 1) / 2, nbr = n - 1;
+  const int64_t chunk = std::min<int64_t>(ncols, 32768);
+  const int64_t ldu = (chunk + 63) / 64 * 64;
+  const int grid = estep_grid(c);
+  std::vector<double> rowsum((size_t)nbr * 64);
+  for (int br = 0; br < nbr; ++br)
+    for (int a = 0; a < 64; ++a) {
+      double t = 0.0;
+      for (int b = 0; b < 64; ++b) t += P[((size_t)br * 64 + a) * 64 + b];
+      rowsum[(size_t)br * 64 + a] = t;
+    }
+  int32_t *d_lc = nullptr, *d_rc = nullptr, *d_leaves = nullptr;
+  double *d_P = nullptr, *d_rs = nullptr, *d_prior = nullptr, *d_w = nullptr, *d_z = nullptr,
+         *d_U = nullptr, *d_V = nullptr, *d_ec = nullptr, *d_scr = nullptr;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) {
+    if (e == hipSuccess) e = x;
+  };
+  chk(dalloc(&d_lc, (size_t)n));
+  chk(dalloc(&d_rc, (size_t)n));
+  chk(dalloc(&d_P, (size_t)nbr * 4096));
+  chk(dalloc(&d_rs, (size_t)nbr * 64));
+  chk(dalloc(&d_prior, 64));
+  chk(dalloc(&d_leaves, (size_t)chunk * nl));
+  if (weights) chk(dalloc(&d_w, (size_t)chunk));
+  chk(dalloc(&d_z, (size_t)chunk));
+  chk(dalloc(&d_U, (size_t)nbr * ldu * 64));
+  chk(dalloc(&d_V, (size_t)nbr * ldu * 64));
+  chk(dalloc(&d_ec, (size_t)nbr * 4096));
+  chk(dalloc(&d_scr, estep_scratch_doubles(n, grid)));
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  auto down = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  };
+  up(d_lc, lc.data(), (size_t)n * 4);
+  up(d_rc, rc.data(), (size_t)n * 4);
+  up(d_P, P, (size_t)nbr * 4096 * 8);
+  up(d_rs, rowsum.data(), rowsum.size() * 8);
+  up(d_prior, prior, 64 * 8);
+  up(d_ec, ecounts, (size_t)nbr * 4096 * 8);  // add_branch_posteriors adds
+  for (int64_t c0 = 0; c0 < ncols && e == hipSuccess; c0 += chunk) {
+    const int64_t nc = std::min(chunk, ncols - c0);
+    up(d_leaves, leaves + c0 * nl, (size_t)nc * nl * 4);
+    if (weights) up(d_w, weights + c0, (size_t)nc * 8);
+    EstepArgs a;
+    a.n = n;
+    a.nl = nl;
+    a.lc = d_lc;
+    a.rc = d_rc;
+    a.P = d_P;
+    a.rowsum = d_rs;
+    a.prior = d_prior;
+    a.leaves = d_leaves;
+    a.weights = d_w;
+    a.ncols = nc;
+    a.z = d_z;
+    a.U = d_U;
+    a.V = d_V;
+    a.ldu = ldu;
+    a.scratch = d_scr;
+    if (e == hipSuccess) e = launch_estep(a, d_P, d_ec, grid, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    down(z + c0, d_z, (size_t)nc * 8);
+  }
+  down(ecounts, d_ec, (size_t)nbr * 4096 * 8);
+  for (auto* q : {d_lc, d_rc, d_leaves}) {
+    int32_t* x = q;
+    dfree(x);
+  }
+  for (auto* q : {d_P, d_rs, d_prior, d_w, d_z, d_U, d_V, d_ec, d_scr}) {
+    double* x = q;
+    dfree(x);
+  }
+  HIP_TRY(c, e);
+  return PCSF_OK;
+}
+
 int lik_impl(pcsf_ctx* c, int k, int n, const int32_t* children, const double* P,
              const double* prior, const int32_t* leaves, const double* leaf_dist, int64_t ncols,
              const double* weights, double* z, int post_mode, double* post, double* ecounts) {
@@ -2000,6 +2087,8 @@ int lik_impl(pcsf_ctx* c, int k, int n, const int32_t* children, const double* P
   if (ncols == 0) return PCSF_OK;
   HIP_TRY(c, hipSetDevice(c->device));
   const bool estep = ecounts != nullptr;
+  if (estep && k == 64 && !dist && post_mode == 0)
+    return estep_mfma(c, n, lc, rc, P, prior, leaves, ncols, weights, z, ecounts);
   const size_t per_col = (size_t)nl * 4 + (dist ? (size_t)nl * k * 8 : 0) + 16 +
                          (post_mode == 2 ? (size_t)n * k * 8 : post_mode == 1 ? (size_t)k * 8 : 0);
   const int64_t chunk =

@@ -275,3 +275,25 @@ struct LikArgs {
 };
 // grid workgroups; with U set, ncols must not exceed grid
 hipError_t launch_lik_posteriors(const LikArgs& a, int grid, hipStream_t s);
+// The E-step on FP64 matrix cores for k = 64 without `Distribution leaves
+// (kernels/estep.hip): u, v per (branch, column), z per column, then
+// ecounts += P o sum_col u v^T per branch.
+struct EstepArgs {
+  int n = 0, nl = 0;
+  const int32_t *lc = nullptr, *rc = nullptr;
+  const double* P = nullptr;        // (n-1) x 64 x 64, row-major per branch
+  const double* rowsum = nullptr;   // (n-1) x 64: P's row sums (`Marginalize leaf messages)
+  const double* prior = nullptr;    // 64
+  const int32_t* leaves = nullptr;  // ncols x nl: state, -1 `Marginalize, >= 64 zeros
+  const double* weights = nullptr;  // ncols (null: 1)
+  int64_t ncols = 0;
+  double* z = nullptr;              // ncols
+  double* U = nullptr;              // (n-1) x ldu x 64
+  double* V = nullptr;
+  int64_t ldu = 0;                  // columns per branch in U, V (>= ncols rounded up to 64)
+  double* scratch = nullptr;        // estep_scratch_doubles(n, grid)
+};
+int estep_grid(const pcsf_ctx* c);
+size_t estep_scratch_doubles(int n, int grid);
+hipError_t launch_estep(const EstepArgs& a, const double* P, double* ecounts, int grid,
+                        hipStream_t s);

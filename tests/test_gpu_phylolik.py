@@ -118,6 +118,51 @@ def test_large_trees_at_64_codons(ctx, params_dir, name):
     assert np.allclose(ss, ec0, rtol=1e-10, atol=1e-13)
 
 
+def test_estep_fast_path_matches_oracle(ctx, params_dir):
+    """The E-step at 64 states without `Distribution leaves runs on the FP64
+    matrix cores (kernels/estep.hip): counts and likelihoods against the
+    oracle with weights, `Marginalize leaves and states beyond the alphabet,
+    over a column count that leaves a partial 64-column block."""
+    _say("E-step fast path, 58mammals")
+    tree, otree = _trees(params_dir, "58mammals")
+    S, pi = O.ecm_read(os.path.join(params_dir, "58mammals_coding.ECM"))
+    Q = O.build_q(S, pi)
+    Pm = np.stack([O.pt(Q, pi, 1.3 * t) for t in otree.branch[:-1]])
+    m = P.PhyloCSFModel(None, tree, [(S, pi)])
+    codes, off = m.simulate([150], [0], [1.3], gap_p=0.2, seed=8)
+    leaves = np.where(codes == 255, -1, codes.astype(np.int32))
+    rng = np.random.default_rng(8)
+    leaves[rng.random(leaves.shape) < 0.01] = 64 + 5  # a vector of zeros
+    w = rng.uniform(0.5, 2.0, leaves.shape[0])
+    z0, _, ec0 = O.lik_posteriors(otree, Pm, pi, leaves, None, w, posteriors=False)
+    ec = np.full((tree.n - 1, 64, 64), 0.5)
+    z = P.collect_sufficient_statistics(ctx, tree, Pm, pi, leaves, ec, weights=w)
+    assert np.allclose(z, z0, rtol=1e-11, atol=0)
+    assert np.allclose(ec, ec0 + 0.5, rtol=1e-10, atol=1e-13)
+
+
+def test_estep_chunks_add_up(ctx, params_dir):
+    """Over more columns than one device chunk (32,768) the counts are the sum
+    of the counts of the parts (the E-step is additive over columns)."""
+    _say("E-step over 40,000 columns = its two parts")
+    tree, otree = _trees(params_dir, "29mammals")
+    S, pi = O.ecm_read(os.path.join(params_dir, "29mammals_coding.ECM"))
+    Q = O.build_q(S, pi)
+    Pm = np.stack([O.pt(Q, pi, t) for t in otree.branch[:-1]])
+    m = P.PhyloCSFModel(None, tree, [(S, pi)])
+    codes, off = m.simulate([40000], [0], [1.0], gap_p=0.1, seed=9)
+    leaves = np.where(codes == 255, -1, codes.astype(np.int32))
+    whole = np.zeros((tree.n - 1, 64, 64))
+    zw = P.collect_sufficient_statistics(ctx, tree, Pm, pi, leaves, whole)
+    parts = np.zeros((tree.n - 1, 64, 64))
+    za = P.collect_sufficient_statistics(ctx, tree, Pm, pi, leaves[:17000], parts)
+    zb = P.collect_sufficient_statistics(ctx, tree, Pm, pi, leaves[17000:], parts)
+    assert np.array_equal(zw, np.concatenate([za, zb]))
+    assert np.allclose(whole, parts, rtol=1e-11, atol=1e-9)
+    # each branch's counts sum to the number of columns (a posterior per column)
+    assert np.allclose(whole.sum(axis=(1, 2)), leaves.shape[0], rtol=1e-9)
+
+
 def test_bad_leaves_refused(ctx, params_dir):
     tree, _ = _trees(params_dir, "12flies")
     Pm = np.tile(np.eye(3), (tree.n - 1, 1, 1))
