@@ -559,7 +559,7 @@ def extra_workload(args, P, W, ctx):
                                       "on 1 thread of %s" % (nr - 1, nc, cdt, cpu_model())})
         return
     # estep
-    wl = W.make("C2", sim, n_regions=args.regions or 60, n_threads=16)
+    wl = W.make("C2", sim, n_regions=args.regions or 600, n_threads=16)
     leaves = np.where(wl.codes == 255, -1, wl.codes.astype(np.int32))
     S, pi = O.ecm_read(prefix + "_coding.ECM")
     Q = O.build_q(S, pi)

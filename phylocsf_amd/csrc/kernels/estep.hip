@@ -160,22 +160,25 @@ __global__ __launch_bounds__(64 * kWaves, 2) void estep_passes_kernel(EstepArgs 
 }
 
 // ecounts_br += P_br o sum_col u v^T over columns [0, ncols) (ncols a
-// multiple of 4; U and V hold ldu columns per branch); one workgroup per
-// branch, wave w the rows 16 w .. 16 w + 15 and all 64 columns (four 16 x 16
-// accumulators)
+// multiple of 4; U and V hold ldu columns per branch).  Workgroup
+// (branch, split) takes the split's share of the columns -- wave w the rows
+// 16 w .. 16 w + 15 and all 64 columns (four 16 x 16 accumulators) -- and
+// adds P o its partial sum with FP64 atomics (P o S is linear in S).
 __global__ __launch_bounds__(256) void estep_counts_kernel(const double* __restrict__ U,
                                                            const double* __restrict__ V,
                                                            int64_t ldu, int64_t ncols,
                                                            const double* __restrict__ P,
                                                            double* __restrict__ ecounts) {
   const int br = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int64_t steps = ncols / 4, per = (steps + gridDim.y - 1) / gridDim.y;
+  const int64_t jb = 4 * per * blockIdx.y, je = std::min<int64_t>(ncols, jb + 4 * per);
   const int i = lane & 15, kk = lane >> 4;
   const double* Ub = U + (size_t)br * ldu * 64;
   const double* Vb = V + (size_t)br * ldu * 64;
   f64x4 acc[4];
 #pragma unroll
   for (int no = 0; no < 4; ++no) acc[no] = (f64x4){0.0, 0.0, 0.0, 0.0};
-  for (int64_t j0 = 0; j0 < ncols; j0 += 4) {
+  for (int64_t j0 = jb; j0 < je; j0 += 4) {
     const double a = Ub[(j0 + kk) * 64 + 16 * w + i];  // A[16 w + i][j0 + kk]
 #pragma unroll
     for (int no = 0; no < 4; ++no)
@@ -190,7 +193,7 @@ __global__ __launch_bounds__(256) void estep_counts_kernel(const double* __restr
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int a = 16 * w + 4 * v + kk, b = 16 * no + i;
-      E[a * 64 + b] += Pb[a * 64 + b] * acc[no][v];
+      atomicAdd(E + a * 64 + b, Pb[a * 64 + b] * acc[no][v]);
     }
 }
 
@@ -212,7 +215,10 @@ hipError_t launch_estep(const EstepArgs& a, const double* P, double* ecounts, in
   hipLaunchKernelGGL(estep_passes_kernel, dim3(g), dim3(64 * kWaves), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(estep_counts_kernel, dim3(a.n - 1), dim3(256), 0, s, a.U, a.V, a.ldu,
-                     nblocks * kCols, P, ecounts);
+  // enough workgroups to fill the device: splits of the columns per branch
+  const int splits = (int)std::max<int64_t>(
+      1, std::min<int64_t>({16, nblocks, (4 * grid + a.n - 2) / (a.n - 1)}));
+  hipLaunchKernelGGL(estep_counts_kernel, dim3(a.n - 1, splits), dim3(256), 0, s, a.U, a.V,
+                     a.ldu, nblocks * kCols, P, ecounts);
   return hipGetLastError();
 }
