@@ -1492,17 +1492,22 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
   // A pair also stops when the Newton step from its last point promises a
   // gain G = h^2 / (2 curv) that leaves nothing to measure, and takes that
   // step.  With the secant's curvature within a factor 1 +- d of lpr's the
-  // step leaves ~d^2 G, d estimated by the change of the secant curvature
-  // between the last two steps (at least 0.1): FP64 asks d^2 G <=
+  // step leaves ~d^2 G, d estimated by the change of the curvature between
+  // the last two steps (the grid parabola's before the first secant; at
+  // least 0.1): FP64 asks d^2 G <=
   // 1e-12 |lpr| (the oracle maximiser's digits), the MFMA precisions
   // d^2 G <= 2e-5 nats (1e-4 dB; their final lpr is FP64's at the estimate).
   const bool strict = fp64_precision(precision);
-  // Per pair: Newton steps on h = d lpr / d ln rho, inside the bracket
-  // [lo, hi] that the signs of h leave for the maximum, with h' from the last
-  // two evaluations (secant) where that is concave, else from the outer
-  // product of the column scores, sum_col (d ln z / d ln rho)^2 (BHHH: the
-  // expected information, whose first step lands near the maximum where a
-  // fixed first step would not).  While the last two points are more than
+  // Per pair: first lpr on a grid of seven tree scales shared by every pair
+  // (ln rho = -2 .. 2, through the fixed-scale path: one transition set per
+  // model and grid point, as find_init's shared abscissae), the start at the
+  // vertex of the parabola through the best grid point and its neighbours,
+  // inside the bracket they span.  Then Newton steps on h = d lpr / d ln rho,
+  // inside the bracket [lo, hi] that the signs of h leave for the maximum,
+  // with h' from the last two evaluations (secant) where that is concave,
+  // else from the parabola's curvature (first step) or the outer product of
+  // the column scores, sum_col (d ln z / d ln rho)^2 (BHHH: the expected
+  // information).  While the last two points are more than
   // 0.01 apart the step goes to the maximum of the cubic through their lpr
   // and h instead (Hermite interpolation: one order above the secant, and
   // the lpr differences still far above their rounding there).  Bisection
@@ -1511,6 +1516,7 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
     bool done = false, have_prev = false, seen_pos = false, seen_neg = false;
     double x = 0, px = 0, ph = 0, pL = 0;
     double pcurv = 0;  // the previous secant curvature (0: none yet)
+    double icurv = 0;  // the grid parabola's curvature (0: none)
     double lo = 0, hi = 0;
     int slow = 0;  // consecutive steps that did not halve |h|
   };
@@ -1520,6 +1526,51 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
     s.hi = xhi;
   }
   st.assign(np, PairState());
+  {
+    constexpr int K = 7;
+    const double grid[K] = {-2.0, -1.2, -0.6, 0.0, 0.6, 1.2, 2.0};
+    std::vector<double> Lk[K], e;
+    // FP64 evaluates the grid on the matrix cores (the plain kernel is the
+    // slow one; both are FP64)
+    const int gp = (precision == PCSF_FP64) ? PCSF_FP64M : precision;
+    for (int k = 0; k < K; ++k) {
+      const int rc = eval_shared(b, std::exp(grid[k]), gp, Lk[k], e);
+      if (rc) return rc;
+    }
+    for (size_t q = 0; q < np; ++q) {
+      st[q].evals += K;
+      int best = 0;
+      bool finite = true;
+      for (int k = 0; k < K; ++k) {
+        finite = finite && std::isfinite(Lk[k][q]);
+        if (Lk[k][q] > Lk[best][q]) best = k;
+      }
+      if (!finite) continue;  // the search starts at rho = 1 over the whole domain
+      G& s = g[q];
+      if (best == 0) {
+        s.x = grid[0];
+        s.hi = grid[1];
+      } else if (best == K - 1) {
+        s.x = grid[K - 1];
+        s.lo = grid[K - 2];
+      } else {
+        const double x0 = grid[best - 1], x1 = grid[best], x2 = grid[best + 1];
+        const double d1 = (Lk[best][q] - Lk[best - 1][q]) / (x1 - x0);
+        const double d2 = (Lk[best + 1][q] - Lk[best][q]) / (x2 - x1);
+        const double a2 = (d2 - d1) / (x2 - x0);  // half the parabola's second derivative
+        s.lo = x0;
+        s.hi = x2;
+        s.seen_pos = s.seen_neg = true;  // lpr rises into x1 and falls after it
+        s.x = x1;
+        if (a2 < 0.0) {
+          const double xv = 0.5 * (x0 + x1) - d1 / (2.0 * a2);
+          if (xv > x0 && xv < x2) s.x = xv;
+          // also the first reference for the secant curvature's change
+          s.icurv = s.pcurv = -2.0 * a2;
+        }
+      }
+    }
+  }
   std::vector<double> rho(np, 1.0), L(np), D(np), S(np);
   std::vector<uint8_t> act(np, 1);
   auto eval = [&]() -> int {
@@ -1591,7 +1642,8 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
       // sum away from the maximum
       const double n_cols = (double)(b->offsets[k / m->nm + 1] - b->offsets[k / m->nm]);
       const double info = std::exp(2.0 * s.x) * S[k] - h * h / std::max(1.0, n_cols);
-      if (!secant && std::isfinite(info) && info > 0.0) curv = info;
+      if (!secant && s.icurv > 0.0 && !s.have_prev) curv = s.icurv;
+      else if (!secant && std::isfinite(info) && info > 0.0) curv = info;
       double xn;
       if (curv > 0.0) {
         xn = s.x + h / curv;
