@@ -1563,10 +1563,42 @@ int score_mle_grad(pcsf_batch* b, int precision, std::vector<PairState>& st) {
         s.seen_pos = s.seen_neg = true;  // lpr rises into x1 and falls after it
         s.x = x1;
         if (a2 < 0.0) {
-          const double xv = 0.5 * (x0 + x1) - d1 / (2.0 * a2);
+          double xv = 0.5 * (x0 + x1) - d1 / (2.0 * a2);
+          double cv = -2.0 * a2;
+          // with a fourth point (the better outer neighbour) the cubic's
+          // maximum and curvature there: one order closer than the parabola's
+          const int o = (best >= 2 && (best + 2 >= K || Lk[best - 2][q] >= Lk[best + 2][q]))
+                            ? best - 2
+                            : (best + 2 < K ? best + 2 : -1);
+          if (o >= 0) {
+            const double x3 = grid[o];
+            const double d3 = (o < best) ? (Lk[best - 1][q] - Lk[o][q]) / (x0 - x3)
+                                         : (Lk[o][q] - Lk[best + 1][q]) / (x3 - x2);
+            // Newton form over x0 < x1 < x2 with x3 outside: p(x) = f0 + d1 (x - x0)
+            // + a2 (x - x0)(x - x1) + a3 (x - x0)(x - x1)(x - x2)
+            const double b2 = (o < best) ? (d1 - d3) / (x1 - x3) : (d3 - d2) / (x3 - x1);
+            const double a3 = (o < best) ? (a2 - b2) / (x2 - x3) : (b2 - a2) / (x3 - x0);
+            // p'(x) = d1 + a2 (2x - x0 - x1) + a3 q'(x), q = (x - x0)(x - x1)(x - x2)
+            auto dp = [&](double x) {
+              const double qd = (x - x1) * (x - x2) + (x - x0) * (x - x2) + (x - x0) * (x - x1);
+              return d1 + a2 * (2.0 * x - x0 - x1) + a3 * qd;
+            };
+            auto d2p = [&](double x) { return 2.0 * a2 + a3 * (6.0 * x - 2.0 * (x0 + x1 + x2)); };
+            // the root of p' in (x0, x2) by Newton from the parabola's vertex
+            double xc = (xv > x0 && xv < x2) ? xv : x1;
+            for (int it = 0; it < 8; ++it) {
+              const double dd = d2p(xc);
+              if (!(dd < 0.0)) break;
+              xc = std::min(std::max(xc - dp(xc) / dd, x0), x2);
+            }
+            if (xc > x0 && xc < x2 && d2p(xc) < 0.0 && std::fabs(dp(xc)) < 1e-6 * std::fabs(cv)) {
+              xv = xc;
+              cv = -d2p(xc);
+            }
+          }
           if (xv > x0 && xv < x2) s.x = xv;
           // also the first reference for the secant curvature's change
-          s.icurv = s.pcurv = -2.0 * a2;
+          s.icurv = s.pcurv = cv;
         }
       }
     }
