@@ -222,11 +222,15 @@ int pcsf_backward(pcsf_batch *b, const double *grad_loglik, double *grad_rho,
  * n_evals = likelihood evaluations per region.  Any output may be NULL. */
 #define PCSF_STRATEGY_FIXED 0
 #define PCSF_STRATEGY_MLE 1
-/* mle with the gradient: root of h = d lpr / d ln rho on [0.01, 10] by
- * Newton steps inside the bracket the signs of h leave -- the first with the
- * column scores' centred outer product as the curvature, then to the
- * maximum of the cubic through the last two points' lpr and h while they
- * are more than 0.01 apart, the secant's Newton point when closer --
+/* mle with the gradient: root of h = d lpr / d ln rho on [0.01, 10].  lpr
+ * of every pair first at seven tree scales shared by all pairs (ln rho =
+ * -2 .. 2, the fixed-scale path); the search starts at the maximum of the
+ * cubic through the best of them and three neighbours, with that cubic's
+ * curvature, inside the bracket of the neighbours.  Then Newton steps inside
+ * the bracket the signs of h leave -- to the maximum of the cubic through
+ * the last two points' lpr and h while they are more than 0.01 apart, the
+ * secant's Newton point when closer, the column scores' centred outer
+ * product as the curvature where neither is known --
  * bisecting when a step leaves the bracket or two steps fail to halve |h|;
  * a pair stops when its next step would leave less than 1e-12 |lpr| (FP64,
  * FP64M) or 2e-5 nats (the MFMA precisions), at 1e-6 (1e-4) in ln rho, or at
