@@ -285,7 +285,8 @@ struct SetModels {
 int ensure_adj_scratch(pcsf_batch* b) {
   const pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
-  const size_t need = adjoint_mfma_scratch_doubles(m, adjoint_mfma_grid(m));
+  const size_t need = adjoint_mfma_scratch_doubles(
+      m, std::max(adjoint_mfma_grid(m, true), adjoint_mfma_grid(m, false)));
   if (b->adj_scratch >= need) return PCSF_OK;
   dfree(b->d_adj_scratch);
   b->adj_scratch = 0;
@@ -370,7 +371,8 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
     if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
     hipError_t e =
         (precision == PCSF_FP64M)
-            ? launch_forward_fp64m(m, b, ch, b->d_adj_scratch, adjoint_mfma_grid(m), c->stream)
+            ? launch_forward_fp64m(m, b, ch, b->d_adj_scratch, adjoint_mfma_grid(m, true),
+                                   c->stream)
         : fp64 ? launch_prune_fp64_regions(m, b, ch, c->stream)
                : launch_prune_mfma_regions(m, b, precision, variant, ch, c->stream);
     if (e == hipErrorInvalidValue && precision == PCSF_FP64)
@@ -446,7 +448,7 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
       rc = ensure_adj_scratch(b);
       if (rc) return rc;
       e = launch_adjoint_mfma(m, b, ch, d_gbr ? d_gbr + (size_t)r0 * nm * m->nbr : nullptr,
-                              b->d_adj_scratch, adjoint_mfma_grid(m), c->stream);
+                              b->d_adj_scratch, adjoint_mfma_grid(m, false), c->stream);
     }
     HIP_TRY(c, e);
     HIP_TRY(c, launch_region_reduce_aligned(b, ch, b->d_gloglik, b->d_grad, c->stream,
@@ -1064,7 +1066,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   all.nblocks = b->al.ld / kAlignCols;
   all.sets = &m->fixed;
   if (precision == PCSF_FP64M) {
-    e = b->nreg > 0 ? launch_forward_fp64m(m, b, all, b->d_adj_scratch, adjoint_mfma_grid(m),
+    e = b->nreg > 0 ? launch_forward_fp64m(m, b, all, b->d_adj_scratch, adjoint_mfma_grid(m, true),
                                            c->stream)
                     : hipSuccess;
   } else if (precision == PCSF_FP64) {

@@ -219,7 +219,7 @@ struct AdjointArgs {
   double* grad_branch;       // (region - r0) x nm x nbr, added to (may be null)
   double* scratch;
 };
-int adjoint_mfma_grid(const pcsf_model* m);
+int adjoint_mfma_grid(const pcsf_model* m, bool forward);
 size_t adjoint_mfma_scratch_doubles(const pcsf_model* m, int grid);
 hipError_t launch_adjoint_mfma(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
                                double* grad_branch, double* scratch, int grid, hipStream_t s);

@@ -214,7 +214,13 @@ size_t adjoint_mfma_scratch_doubles(const pcsf_model* m, int grid) {
   return (size_t)grid * kWaves * (m->n - m->nl) * kTileDoubles;
 }
 
-int adjoint_mfma_grid(const pcsf_model* m) { return m->ctx->num_cus * 2; }
+// resident workgroups per CU: two for both kernels (the outside pass: 243
+// registers and 64 KiB of LDS; the forward: 220 and two 32 KiB P buffers --
+// a single-buffered forward at three per CU measured no faster)
+int adjoint_mfma_grid(const pcsf_model* m, bool forward) {
+  (void)forward;
+  return m->ctx->num_cus * 2;
+}
 
 namespace {
 
