@@ -97,13 +97,25 @@ struct ProgramBuilder {
       w1 = br | (nbr << 16);
     }
     // the last GEMM before a gather op (with no gather op in between) issues
-    // that op's first leaf gather
+    // that op's first leaf gather or its table row, unless the GEMM runs at
+    // the deepest GEMM level (the tile issued ahead is live through the
+    // GEMM, so the program's peak of live tiles stays where it was)
+    int deepest = -1;
+    for (int g : gemm_pos) deepest = std::max(deepest, (ops[kOpWords * g] >> 8) & 0xff);
     int last_gemm = -1;
     for (int k = 0; k < count(); ++k) {
       const int op = opcode(k);
       if (op == OP_GEMM) {
-        last_gemm = k;
+        last_gemm = ((ops[kOpWords * k] >> 8) & 0xff) < deepest ? k : -1;
       } else if (op == OP_CHERRYT) {
+        const int a0 = ops[kOpWords * k + 1] & 0xffff, a1 = (ops[kOpWords * k + 1] >> 16) & 0xffff;
+        const int tab = ops[kOpWords * k + 2];
+        if (last_gemm >= 0 && a0 <= kPfLeafMax && a1 <= kPfLeafMax && tab <= kPfTableMax) {
+          ops[kOpWords * last_gemm + 3] = static_cast<int32_t>(
+              kPfTable | (static_cast<uint32_t>(tab) << 24) | (static_cast<uint32_t>(a1) << 12) |
+              static_cast<uint32_t>(a0));
+          ops[kOpWords * k] |= 1 << 16;
+        }
         last_gemm = -1;
       } else if (op == OP_CHERRY || op == OP_LEAFMUL) {
         if (last_gemm >= 0) {

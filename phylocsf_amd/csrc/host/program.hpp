@@ -4,8 +4,10 @@
 // partial-likelihood tiles live in registers.  Four words per op:
 // word0 = opcode | depth << 8 | use_prefetch << 16, word1 = arg0 | arg1 << 16,
 // word2 = leaf whose gather a GEMM issues ahead (kNoBranch if none) or the
-// table index of OP_CHERRYT, word3 = 0.  use_prefetch marks a gather op whose
-// first leaf (arg0) was issued by the preceding GEMM.  This is this
+// table index of OP_CHERRYT, word3 = 0 or, for a GEMM, the table row it
+// issues ahead: kPfTable | table << 24 | leaf1 << 12 | leaf0.  use_prefetch
+// marks a gather op whose first leaf (arg0), or whose table row, was issued
+// by the preceding GEMM.  This is this
 // implementation's schedule of PhyloLik.ensure_alpha
 // (lib/CamlPaml/PhyloLik.ml:78-90); the reference has no counterpart.
 #pragma once
@@ -26,6 +28,8 @@ constexpr int kTabCodes = 65;  // table index per leaf: codon 0..63, 64 = `Margi
 constexpr int kNoBranch = 0xFFFF;
 constexpr int kOpWords = 4;
 constexpr int kMaxSlots = 6;
+constexpr uint32_t kPfTable = 0x80000000u;  // word3 of a GEMM: a table row is issued ahead
+constexpr int kPfLeafMax = 0xfff, kPfTableMax = 0x7f;
 
 namespace pcsf {
 

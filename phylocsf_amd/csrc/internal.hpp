@@ -243,7 +243,8 @@ hipError_t launch_region_reduce_aligned(const pcsf_batch* b, const RegionChunk& 
 struct MfmaVariant {
   int wpg;        // waves per workgroup (16 columns each) sharing one LDS P tile
   int minw;       // minimum waves per SIMD the compiler must allow (register cap)
-  bool prefetch;  // GEMMs issue the next leaf gather ahead (+18 VGPRs)
+  bool prefetch;  // GEMMs issue the next gather op's leaf or table row ahead (+18
+                  // VGPRs; the program names them only at GEMMs above its deepest)
   int g;          // 16-column tiles per wave, walked side by side
   const char* what;
 };

@@ -16,7 +16,7 @@ const MfmaVariant kMfmaVariants[kNumMfmaVariants] = {
     {4, 3, false, 1, "4 waves/WG, >=3 waves/SIMD"},
     {2, 2, true, 1, "2 waves/WG, >=2 waves/SIMD, gather prefetch"},
     {4, 2, false, 1, "4 waves/WG, >=2 waves/SIMD"},
-    {2, 3, false, 1, "2 waves/WG, >=3 waves/SIMD"},
+    {4, 3, true, 1, "4 waves/WG, >=3 waves/SIMD, gather prefetch above the deepest GEMMs"},
     {2, 1, false, 2, "2 waves/WG, 2 column tiles per wave"},
 };
 

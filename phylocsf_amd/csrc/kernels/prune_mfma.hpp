@@ -250,25 +250,67 @@ struct MaskG {  // per group: lanes whose column is a constant vector
   uint64_t m[G];
 };
 
-// acc = offdiag(P) s in half precision for the tile staged in LDS (Ab) and
-// each group's 64x16 tile; F16S leaves the cross terms hi*lo + lo*hi, which
-// carry 2^kHalfExp, in accx.  The tile is kept at 2^kHalfExp above its GEMM
-// operand (FP16: operand maximum in [2^12, 2^13)), so the product comes out
-// at the tile's scale, and hi = RN(s 2^-13) and the residual RN(s - 2^13 hi)
-// are one mixed-precision FMA each.  Both accumulators start from zero (the
-// matrix cores' FP32 accumulation rounds toward zero, which would bias s if
-// it were the initial value).  Tile layout of the halves (pbuild.hip): hi fragments, then lo fragments,
-// 512 x 8 values each, in the A-operand order of
-// v_mfma_f32_16x16x32_{f16,bf16} with K permuted so that K-step st of a lane
-// in group g takes rows 16*(2*st + j/4) + 4*g + j%4 of its column: exactly
-// what the accumulator layout leaves in registers s.r[2*st + j/4][j%4].
-// PRIO raises the wave's issue priority over its MFMA burst: with three waves
-// per SIMD the arbiter then keeps the matrix pipe fed ahead of the other
-// waves' rescaling and combining (measured: 6% on the 58mammals forward at
-// that shape, a few percent slower at two waves per SIMD, hence per shape).
+// hi = RN(x 2^-13) and lo = RN(x - 2^13 hi) of eight FP32 values (a, then
+// b), packed in pairs: v_fma_mixlo_f16 / v_fma_mixhi_f16 round one FMA
+// straight into the low / high half of a register -- x 2^-13 + 0 and
+// hi (-2^13) + x, both exact in FP32 before their one rounding, so the
+// halves are those of a multiply and v_cvt_pk_f16_f32 (which the compiler
+// emits instead: three instructions per pair for hi and lo each, against
+// two).  The hazard recognizer does not see an assembly block as a VALU
+// write, so the block ends with the wait states such a write needs before a
+// matrix instruction reads its register.
+__device__ __forceinline__ void split_f16x8(const f32x4& a, const f32x4& b, f16x8& hi,
+                                            f16x8& lo) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  u32x4 h, l;
+  asm("v_fma_mixlo_f16 %0, %8, %16, 0 op_sel_hi:[0,0,0]\n\t"
+      "v_fma_mixhi_f16 %0, %9, %16, 0 op_sel_hi:[0,0,0]\n\t"
+      "v_fma_mixlo_f16 %1, %10, %16, 0 op_sel_hi:[0,0,0]\n\t"
+      "v_fma_mixhi_f16 %1, %11, %16, 0 op_sel_hi:[0,0,0]\n\t"
+      "v_fma_mixlo_f16 %2, %12, %16, 0 op_sel_hi:[0,0,0]\n\t"
+      "v_fma_mixhi_f16 %2, %13, %16, 0 op_sel_hi:[0,0,0]\n\t"
+      "v_fma_mixlo_f16 %3, %14, %16, 0 op_sel_hi:[0,0,0]\n\t"
+      "v_fma_mixhi_f16 %3, %15, %16, 0 op_sel_hi:[0,0,0]\n\t"
+      "v_fma_mixlo_f16 %4, %0, %17, %8 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %4, %0, %17, %9 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixlo_f16 %5, %1, %17, %10 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %5, %1, %17, %11 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixlo_f16 %6, %2, %17, %12 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %6, %2, %17, %13 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixlo_f16 %7, %3, %17, %14 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %7, %3, %17, %15 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
+      "s_nop 4"
+      : "=&v"(h[0]), "=&v"(h[1]), "=&v"(h[2]), "=&v"(h[3]), "=&v"(l[0]), "=&v"(l[1]),
+        "=&v"(l[2]), "=&v"(l[3])
+      : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(b[0]), "v"(b[1]), "v"(b[2]), "v"(b[3]),
+        "s"(kHalfUnscale), "s"(-(float)(1 << kHalfExp)));
+  hi = __builtin_bit_cast(f16x8, h);
+  lo = __builtin_bit_cast(f16x8, l);
+}
+
+// s = P s in half precision for the tile staged in LDS (Ab) and each group's
+// 64x16 tile.  The tile is kept at 2^kHalfExp above its GEMM operand (FP16:
+// operand maximum in [2^12, 2^13)), so the product comes out at the tile's
+// scale, and hi = RN(s 2^-13) and the residual RN(s - 2^13 hi) are one
+// mixed-precision FMA each.  Per 16-row block mo: hi*hi into acc and (F16S)
+// the cross terms hi*lo + lo*hi, which carry 2^kHalfExp, into accx, both from
+// zero (the matrix cores' FP32 accumulation rounds toward zero, which would
+// bias s if it were the initial value); then the block's message
+// m = s + fma(P_aa - 1, s, acc + 2^-13 accx) in round-to-nearest VALU
+// arithmetic, which runs while the next block's MFMAs do.  Columns whose
+// partial is a constant vector (mask k) keep it.  Tile layout of the halves
+// (pbuild.hip): hi fragments, then lo fragments, 512 x 8 values each, in the
+// A-operand order of v_mfma_f32_16x16x32_{f16,bf16} with K permuted so that
+// K-step st of a lane in group g takes rows 16*(2*st + j/4) + 4*g + j%4 of
+// its column: exactly what the accumulator layout leaves in registers
+// s.r[2*st + j/4][j%4].  PRIO raises the wave's issue priority over its MFMA
+// burst: with three waves per SIMD the arbiter then keeps the matrix pipe fed
+// ahead of the other waves' rescaling and combining (measured: 6% on the
+// 58mammals forward at that shape, a few percent slower at two waves per
+// SIMD, hence per shape).
 template <int PREC, int G, bool PRIO>
-__device__ __forceinline__ void gemm_half(f32x4 (&acc)[G][4], f32x4 (&accx)[G][4],
-                                          const TileG<G>& s, const float* Ab, int lane) {
+__device__ __forceinline__ void gemm_half(TileG<G>& s, const MaskG<G>& k, const float* Ab,
+                                          int lane, int grp) {
   constexpr bool kBF = PREC == kPrecBF16;
   using V8 = typename std::conditional<kBF, bf16x8, f16x8>::type;
   using E = typename std::conditional<kBF, __bf16, _Float16>::type;
@@ -278,17 +320,12 @@ __device__ __forceinline__ void gemm_half(f32x4 (&acc)[G][4], f32x4 (&accx)[G][4
   for (int c = 0; c < G; ++c) {
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        bh[c][st][j] = (E)(s.g[c].r[2 * st + (j >> 2)][j & 3] * kDown);
       if constexpr (PREC == kPrecF16S) {
-        // the residual is taken against the rounded hi in its register (the
-        // compiler would otherwise recompute hi inside each residual FMA)
-        asm("" : "+v"(bh[c][st]));
+        split_f16x8(s.g[c].r[2 * st], s.g[c].r[2 * st + 1], bh[c][st], bl[c][st]);
+      } else {
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-          bl[c][st][j] = (E)fmaf((float)bh[c][st][j], -(float)(1 << kHalfExp),
-                                 s.g[c].r[2 * st + (j >> 2)][j & 3]);
+          bh[c][st][j] = (E)(s.g[c].r[2 * st + (j >> 2)][j & 3] * kDown);
       }
     }
   }
@@ -298,37 +335,48 @@ __device__ __forceinline__ void gemm_half(f32x4 (&acc)[G][4], f32x4 (&accx)[G][4
     if constexpr (kBF) return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, cc, 0, 0, 0);
     else return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, cc, 0, 0, 0);
   };
-#pragma unroll
-  for (int c = 0; c < G; ++c)
-#pragma unroll
-    for (int mo = 0; mo < 4; ++mo) {
-      acc[c][mo] = (f32x4)(0.0f);
-      if constexpr (PREC == kPrecF16S) accx[c][mo] = (f32x4)(0.0f);
-    }
-  // one K-step at a time, so that at most 8 A fragments (32 VGPRs) are live
+  TileG<G> m;
   if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
-  for (int st = 0; st < 2; ++st) {
-    V8 ah[4], al[4];
+  for (int mo = 0; mo < 4; ++mo) {
+    V8 ah[2], al[2];
 #pragma unroll
-    for (int mo = 0; mo < 4; ++mo) {
+    for (int st = 0; st < 2; ++st) {
       const int f = (mo * 2 + st) * 64 + lane;
-      ah[mo] = Ah[f];
-      if constexpr (PREC == kPrecF16S) al[mo] = Al[f];
+      ah[st] = Ah[f];
+      if constexpr (PREC == kPrecF16S) al[st] = Al[f];
     }
+    const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
 #pragma unroll
-    for (int mo = 0; mo < 4; ++mo)
+    for (int c = 0; c < G; ++c) {
+      f32x4 acc = (f32x4)(0.0f), accx = (f32x4)(0.0f);
 #pragma unroll
-      for (int c = 0; c < G; ++c) {
+      for (int st = 0; st < 2; ++st) {
         if constexpr (PREC == kPrecF16S) {  // cross terms, carrying 2^kHalfExp
-          accx[c][mo] = mma(al[mo], bh[c][st], accx[c][mo]);
-          accx[c][mo] = mma(ah[mo], bl[c][st], accx[c][mo]);
+          accx = mma(al[st], bh[c][st], accx);
+          accx = mma(ah[st], bl[c][st], accx);
         }
-        acc[c][mo] = mma(ah[mo], bh[c][st], acc[c][mo]);
+        acc = mma(ah[st], bh[c][st], acc);
       }
-    if (st == 0) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float u = acc[v];
+        if constexpr (PREC == kPrecF16S) u = fmaf(accx[v], kHalfUnscale, u);
+        m.g[c].r[mo][v] = s.g[c].r[mo][v] + fmaf(dd[v], s.g[c].r[mo][v], u);
+      }
+    }
   }
   if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
+#pragma unroll
+  for (int c = 0; c < G; ++c) {
+    if (k.m[c] == 0) {  // no lane holds a constant column (wave-uniform)
+      s.g[c] = m.g[c];
+    } else {
+      const bool cst = (k.m[c] >> lane) & 1;
+#pragma unroll
+      for (int mo = 0; mo < 4; ++mo) s.g[c].r[mo] = cst ? s.g[c].r[mo] : m.g[c].r[mo];
+    }
+  }
 }
 
 // One wave's walk of the op program over its G x 16 columns, and the ops.
@@ -351,6 +399,7 @@ struct Walk {
   L3 (*Pbuf3)[kTileFloats] = nullptr;  // the same, as LDS addresses
   const uint8_t* codes_s;      // LDS: the group's leaf codes, nl x GC
   int tid, lane, grp;
+  int wave_s = 0;              // the wave's index in the workgroup (scalar)
   int ccol[G];                 // this lane's columns within the workgroup's GC
   int q = 0, set = 0;          // model and transition set of the current task
   int64_t col[G];              // this lane's columns
@@ -359,7 +408,8 @@ struct Walk {
   int kg = 0;                  // GEMMs done (selects the LDS buffer)
   int E[G];                    // sums of the power-of-two rescaling exponents
   double LS[G];                // sums of ln P[c][c] of the gathered leaves
-  Gathered PF[G];              // leaf gathers issued ahead by the previous GEMM
+  Gathered PF[G];              // leaf gathers or table rows issued ahead by the previous GEMM
+  double LSpf[G];              // log factor of a table row issued ahead
 
   __device__ __forceinline__ Walk(const MfmaParams& prm, float (*pb)[kTileFloats],
                                   const uint8_t* cs, int t, int l, int g, int cc0)
@@ -370,6 +420,7 @@ struct Walk {
       col[c] = 0;
       E[c] = 0;
       LS[c] = 0.0;
+      LSpf[c] = 0.0;
       PF[c].marg = true;
     }
   }
@@ -398,10 +449,13 @@ struct Walk {
       mul_rescale<KE>(s.g[c], leaf_tile(g1), E[c]);
     }
   }
-  // s = message above the cherry (a0, a1), from table `tab`
-  __device__ __forceinline__ void cherryt(T& s, M& k, int a0, int a1, int tab) {
+  // Loads row (a0, a1) of this set's cherry table `tab` for every column
+  // tile: the message into g[c].t, its log factor into ls[c]; m[c] = both
+  // leaves marginalised.
+  __device__ __forceinline__ void table_row(int st, int a0, int a1, int tab, Gathered (&g)[G],
+                                            double (&ls)[G]) {
     constexpr uint32_t kPairs = kTabCodes * kTabCodes;
-    const size_t t0 = (size_t)oset() * p.nctab * kPairs;  // this set's first table entry
+    const size_t t0 = (size_t)st * p.nctab * kPairs;  // this set's first table entry
     const Rsrc rt = make_rsrc(p.ctab + t0 * 64, p.nctab * kPairs * 256u);
     const Rsrc rl = make_rsrc(p.clnt + t0, p.nctab * kPairs * 8u);
 #pragma unroll
@@ -411,13 +465,24 @@ struct Walk {
       const uint32_t e = (uint32_t)(ma ? 64 : ca) * kTabCodes + (mb ? 64 : cb);
 #pragma unroll
       for (int mbk = 0; mbk < 4; ++mbk)
-        s.g[c].r[mbk] =
+        g[c].t.r[mbk] =
             bload4(rt, e * 256u + (uint32_t)grp * 16u + 64u * mbk, tab * kPairs * 256u);
+      ls[c] = bload_f64(rl, e * 8u, tab * kPairs * 8u);
+      g[c].marg = ma && mb;
+    }
+  }
+  // s = message above the cherry (a0, a1), from table `tab`: its row issued
+  // ahead by the preceding GEMM (use_pf), or loaded now
+  __device__ __forceinline__ void cherryt(T& s, M& k, int a0, int a1, int tab, bool use_pf) {
+    if (!(PREF && use_pf)) table_row(oset(), a0, a1, tab, PF, LSpf);
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      s.g[c] = PF[c].t;
       // (a table message always meets a sibling in LEAFMUL / MULPOP, whose
       // rescaling sets the tile scale, before it reaches a GEMM)
-      LS[c] += bload_f64(rl, e * 8u, tab * kPairs * 8u);
+      LS[c] += LSpf[c];
       asm("" : "+v"(LS[c]));  // as E in mul_rescale
-      k.m[c] = __ballot(ma && mb);
+      k.m[c] = __ballot(PF[c].marg);
     }
   }
   // s = s o leaf a0 (rescaled)
@@ -438,14 +503,19 @@ struct Walk {
     }
   }
   // s = P s for the P tile staged in LDS; prefetches the tile of branch
-  // next_br (or the next task's first tile) and, if pf_leaf names one, the
-  // next leaf gather
-  __device__ __forceinline__ void gemm(T& s, const M& k, int next_br, int pf_leaf) {
+  // next_br (or the next task's first tile) and, if the program names one,
+  // the next gather op's first leaf (pf_leaf) or table row (pf_tab:
+  // kPfTable | table << 24 | leaf1 << 12 | leaf0)
+  __device__ __forceinline__ void gemm(T& s, const M& k, int next_br, int pf_leaf,
+                                       uint32_t pf_tab) {
     __syncthreads();  // Pbuf[kg&1] complete; Pbuf[(kg&1)^1] no longer read
     const int st = oset();
     if (PREF && pf_leaf != kNoBranch) {
 #pragma unroll
       for (int c = 0; c < G; ++c) gather(PF[c], p, codes_s, GC, st, pf_leaf, ccol[c], grp);
+    } else if (PREF && (pf_tab & kPfTable)) {
+      table_row(st, pf_tab & kPfLeafMax, (pf_tab >> 12) & kPfLeafMax, (pf_tab >> 24) & kPfTableMax,
+                PF, LSpf);
     }
     // the next tile, of this set (buffer loads from the set's tiles) or the
     // next task's first (plain loads, once per task), goes straight from
@@ -463,7 +533,7 @@ struct Walk {
 #pragma unroll
         for (int q4 = 0; q4 < PER; ++q4)
           __builtin_amdgcn_raw_ptr_buffer_load_lds(
-              r, (LdsPtr)(dst + ((tid & ~63) + NT * q4) * 4), 16, (uint32_t)tid * 16u,
+              r, (LdsPtr)(dst + (64 * wave_s + NT * q4) * 4), 16, (uint32_t)tid * 16u,
               so + 16u * NT * q4, 0, 0);
         if (tid < 16)
           __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (LdsPtr)(dst + 4096), 16,
@@ -474,15 +544,15 @@ struct Walk {
 #pragma unroll
         for (int q4 = 0; q4 < PER; ++q4)
           __builtin_amdgcn_global_load_lds((GPtr)(nsrc + (tid + NT * q4) * 4),
-                                           (LdsPtr)(dst + ((tid & ~63) + NT * q4) * 4), 16, 0, 0);
+                                           (LdsPtr)(dst + (64 * wave_s + NT * q4) * 4), 16, 0, 0);
         if (tid < 16)
           __builtin_amdgcn_global_load_lds((GPtr)(nsrc + 4096 + tid * 4), (LdsPtr)(dst + 4096),
                                            16, 0, 0);
       }
     }
     const float* Ab = Pbuf[kg & 1];
-    f32x4 acc[G][4], accx[G][4];
     if constexpr (PREC == kPrecF32) {
+      f32x4 acc[G][4];
 #pragma unroll
       for (int c = 0; c < G; ++c)
 #pragma unroll
@@ -528,33 +598,7 @@ struct Walk {
         }
       }
     } else {
-      gemm_half<PREC, G, PRIO>(acc, accx, s, Ab, lane);
-      // m = alpha + (P_aa - 1) alpha + offdiag(P) alpha, as in FP32: the
-      // half-precision products sum nonnegative terms only, the diagonal
-      // and the identity are round-to-nearest VALU arithmetic
-#pragma unroll
-      for (int mo = 0; mo < 4; ++mo) {
-        const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
-#pragma unroll
-        for (int c = 0; c < G; ++c)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            float u = acc[c][mo][v];
-            if constexpr (PREC == kPrecF16S) u = fmaf(accx[c][mo][v], kHalfUnscale, u);
-            acc[c][mo][v] = s.g[c].r[mo][v] + fmaf(dd[v], s.g[c].r[mo][v], u);
-          }
-      }
-#pragma unroll
-      for (int c = 0; c < G; ++c) {
-        if (k.m[c] == 0) {  // no lane holds a constant column (wave-uniform)
-#pragma unroll
-          for (int mo = 0; mo < 4; ++mo) s.g[c].r[mo] = acc[c][mo];
-        } else {
-          const bool cst = (k.m[c] >> lane) & 1;
-#pragma unroll
-          for (int mo = 0; mo < 4; ++mo) s.g[c].r[mo] = cst ? s.g[c].r[mo] : acc[c][mo];
-        }
-      }
+      gemm_half<PREC, G, PRIO>(s, k, Ab, lane, grp);
     }
     ++kg;
   }
@@ -598,6 +642,7 @@ struct Interp {
       const int w0 = __builtin_amdgcn_readfirstlane(prog[4 * pc]);
       const int w1 = __builtin_amdgcn_readfirstlane(prog[4 * pc + 1]);
       const int w2 = __builtin_amdgcn_readfirstlane(prog[4 * pc + 2]);
+      const uint32_t w3 = __builtin_amdgcn_readfirstlane(prog[4 * pc + 3]);
       const int op = w0 & 0xff, d = (w0 >> 8) & 0xff;
       const bool use_pf = (w0 >> 16) & 1;
       const int a0 = w1 & 0xffff, a1 = (w1 >> 16) & 0xffff;
@@ -607,13 +652,13 @@ struct Interp {
     if constexpr (D < NS) w.cherry(S[D], K[D], a0, a1, use_pf); \
     break;                                                     \
   case OP_GEMM * 8 + D:                                        \
-    if constexpr (D < NS) w.gemm(S[D], K[D], a1, w2 & 0xffff); \
+    if constexpr (D < NS) w.gemm(S[D], K[D], a1, w2 & 0xffff, w3); \
     break;                                                     \
   case OP_LEAFMUL * 8 + D:                                     \
     if constexpr (D < NS) w.leafmul(S[D], K[D], a0, use_pf);    \
     break;                                                     \
   case OP_CHERRYT * 8 + D:                                     \
-    if constexpr (D < NS) w.cherryt(S[D], K[D], a0, a1, w2);   \
+    if constexpr (D < NS) w.cherryt(S[D], K[D], a0, a1, w2, use_pf); \
     break;                                                     \
   case OP_MULPOP * 8 + D:                                      \
     if constexpr (D + 1 < NS) w.mulpop(S[D], K[D], S[D + 1], K[D + 1]); \
@@ -643,7 +688,7 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
   static_assert(NT >= 16, "the diagonal (16 float4) is copied by the first 16 threads");
   __shared__ __attribute__((aligned(16))) float Pbuf[2][kTileFloats];
   extern __shared__ __attribute__((aligned(16))) uint8_t codes_s[];  // nl x GC
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int grp = lane >> 4, lcol = lane & 15;
 
   TileG<G> S[NS];
@@ -674,6 +719,7 @@ __global__ __launch_bounds__(64 * WPG, MINW) void prune_mfma_kernel(MfmaParams p
   // wave w holds columns [16 G w, 16 G (w + 1)) of the workgroup's group
   W w(p, Pbuf, codes_s, tid, lane, grp, wave * 16 * G + lcol);
   w.Pbuf3 = (typename W::L3 (*)[kTileFloats])Pbuf;
+  w.wave_s = wave;
   for (;;) {
     Task nxt;
     int64_t jn = jc + 1;
@@ -747,7 +793,7 @@ hipError_t launch_variant(const MfmaParams& prm, int variant, int slots, int cus
     case 2: return launch_v<PREC, 4, 3, false>(prm, slots, 0, cus, s);
     case 3: return launch_v<PREC, 2, 2, true>(prm, slots, 0, cus, s);
     case 4: return launch_v<PREC, 4, 2, false>(prm, slots, 0, cus, s);
-    case 5: return launch_v<PREC, 2, 3, false>(prm, slots, 0, cus, s);
+    case 5: return launch_v<PREC, 4, 3, true>(prm, slots, 0, cus, s);
     case 6: return launch_v<PREC, 2, 1, false, 2>(prm, slots, 0, cus, s);
     default: return hipErrorInvalidValue;
   }

@@ -117,7 +117,7 @@ struct TWalk {
   }
   // programs with cherry tables never reach this kernel (a table message has
   // no tangent row); the op exists so the program drivers compile
-  __device__ __forceinline__ void cherryt(TTile&, uint64_t&, int, int, int) {}
+  __device__ __forceinline__ void cherryt(TTile&, uint64_t&, int, int, int, bool) {}
   __device__ __forceinline__ void leafmul(TTile& x, uint64_t& k, int a0, bool) {
     TGathered g;
     tgather(g, a0);
@@ -140,7 +140,7 @@ struct TWalk {
   }
   // m = P s, m' = P t + D s for the tiles staged in LDS; the next P and D
   // tiles go straight from memory into the idle LDS buffers (as in Walk::gemm)
-  __device__ __forceinline__ void gemm(TTile& x, uint64_t k, int next_br, int) {
+  __device__ __forceinline__ void gemm(TTile& x, uint64_t k, int next_br, int, uint32_t) {
     __syncthreads();  // tiles of buffer kg&1 complete; the other buffers are free
     {
       typedef __attribute__((address_space(3))) void* LdsPtr;

@@ -36,6 +36,7 @@ hipError_t launch_tree_prec(const MfmaParams& prm, int variant, int cus, hipStre
     case 1: return launch_ns<PREC, PROG::NS, 4, 2, true, PROG>(prm, 0, cus, s);
     case 2: return launch_ns<PREC, PROG::NS, 4, 3, false, PROG>(prm, 0, cus, s);
     case 4: return launch_ns<PREC, PROG::NS, 4, 2, false, PROG>(prm, 0, cus, s);
+    case 5: return launch_ns<PREC, PROG::NS, 4, 3, true, PROG>(prm, 0, cus, s);
     case 6: return launch_ns<PREC, PROG::NS, 2, 1, false, PROG, 2>(prm, 0, cus, s);
     default: return hipErrorNotSupported;
   }

@@ -75,3 +75,46 @@ def test_program_matches_oracle(name, tables, params_dir):
     for c in range(codes.shape[0]):
         z = interpret(ops, tree, Pm, pi, codes[c], cherries)
         assert abs(z - z_ref[c]) <= 1e-12 * z_ref[c], (name, tables, c)
+
+
+PF_TABLE = 0x80000000
+
+
+@pytest.mark.parametrize("name", ["12flies", "58mammals", "100vertebrates", "120mammals"])
+@pytest.mark.parametrize("tables", [False, True])
+def test_prefetch_words_name_the_next_gather(name, tables, params_dir):
+    """A GEMM's word 2 (leaf) or word 3 (table row) names exactly the first
+    gather op after it, which carries the use-prefetch bit; no other op
+    carries it, and no GEMM at the deepest GEMM level issues one."""
+    tree = P.newick_tree(open(os.path.join(params_dir, name + ".nh")).read())
+    ops, _ = program(tree, tables)
+    ops = ops.astype(np.int64) & 0xFFFFFFFF
+    opc = ops[:, 0] & 0xFF
+    depth = (ops[:, 0] >> 8) & 0xFF
+    deepest = depth[opc == OP_GEMM].max()
+    claimed = set()
+    n_pf = 0
+    for k in np.nonzero(opc == OP_GEMM)[0]:
+        leaf, row = ops[k, 2] & 0xFFFF, ops[k, 3]
+        if leaf == 0xFFFF and not row & PF_TABLE:
+            continue
+        assert depth[k] < deepest, (k, depth[k])
+        assert leaf == 0xFFFF or not row & PF_TABLE
+        j = k + 1
+        while opc[j] not in (OP_CHERRY, OP_LEAFMUL, OP_CHERRYT):
+            assert opc[j] != OP_GEMM and opc[j] != OP_END, (k, j)
+            j += 1
+        assert (ops[j, 0] >> 16) & 1, (k, j)
+        a0, a1 = ops[j, 1] & 0xFFFF, (ops[j, 1] >> 16) & 0xFFFF
+        if leaf != 0xFFFF:
+            assert opc[j] in (OP_CHERRY, OP_LEAFMUL) and a0 == leaf
+        else:
+            assert opc[j] == OP_CHERRYT
+            assert (row & 0xFFF, (row >> 12) & 0xFFF, (row >> 24) & 0x7F) == (a0, a1, ops[j, 2])
+        claimed.add(j)
+        n_pf += 1
+    flagged = set(np.nonzero((ops[:, 0] >> 16) & 1)[0])
+    assert flagged == claimed
+    assert n_pf > 0
+    if tables:
+        assert any(opc[j] == OP_CHERRYT for j in claimed)
