@@ -78,6 +78,7 @@ hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0,
   chk(dalloc(&ps.d_rho, (size_t)cap));
   chk(dalloc(&ps.d_active, (size_t)cap));
   if (fp64) chk(dalloc(&ps.d_P64T, n * 4096));
+  if (fp64) chk(dalloc(&ps.d_P64F, n * kTile64));
   if (tangent) {
     chk(dalloc(&ps.d_DT32, n * kLeafRows * 64));
     chk(dalloc(&ps.d_Dfrag, n * 4096));
@@ -105,6 +106,7 @@ hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0,
 
 void free_psets(PSets& ps) {
   dfree(ps.d_P64T);
+  dfree(ps.d_P64F);
   dfree(ps.d_PT32);
   dfree(ps.d_Pfrag);
   dfree(ps.d_lnPcc);
@@ -166,7 +168,8 @@ int pbuild_failure(pcsf_model* m, unsigned long long key, const double* rho_host
 // branch, row) in the reference's order; its message is made on the host
 // from the same P(t) built in FP64 by Q.Diag.real_to_Pt's rules.
 int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int fmt,
-               float* ms_out, unsigned outputs = kBuildP64 | kBuildMfma | kBuildTangent) {
+               float* ms_out,
+               unsigned outputs = kBuildP64 | kBuildP64M | kBuildMfma | kBuildTangent) {
   pcsf_ctx* c = m->ctx;
   HIP_TRY(c, hipMemsetAsync(m->d_perr, 0xff, sizeof(unsigned long long), c->stream));
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -269,7 +272,8 @@ int ensure_aligned(pcsf_batch* b) {
 // Bytes of one transition set on the device.
 size_t set_bytes(const pcsf_model* m, bool fp64, bool tangent = false, bool own_model = false) {
   return (size_t)m->nbr * (kLeafRows * 64 * 4 + kTileFloats * 4 + kLeafRows * 8 +
-                           (fp64 ? 4096 * 8 : 0) + (tangent ? (kLeafRows + 64) * 64 * 4 : 0)) +
+                           (fp64 ? (4096 + kTile64) * 8 : 0) +
+                           (tangent ? (kLeafRows + 64) * 64 * 4 : 0)) +
          (own_model ? (2 * 4096 + 3 * 64) * 8 : 0);
 }
 
@@ -280,13 +284,12 @@ struct SetModels {
   const double* qs;
 };
 
-// Node tiles of the FP64 matrix-core kernel (forward and outside pass): one
+// Node tiles of the FP64 outside pass (adjoint_mfma.hip): one
 // set per resident wave, allocated on first use.
 int ensure_adj_scratch(pcsf_batch* b) {
   const pcsf_model* m = b->m;
   pcsf_ctx* c = m->ctx;
-  const size_t need = adjoint_mfma_scratch_doubles(
-      m, std::max(adjoint_mfma_grid(m, true), adjoint_mfma_grid(m, false)));
+  const size_t need = adjoint_mfma_scratch_doubles(m, adjoint_mfma_grid(m));
   if (b->adj_scratch >= need) return PCSF_OK;
   dfree(b->d_adj_scratch);
   b->adj_scratch = 0;
@@ -310,10 +313,6 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
   const int fmt = fp64 ? kFragF32 : frag_format(precision);
   const size_t budget = (size_t)8 << 30;
   const bool own = sm != nullptr;
-  if (precision == PCSF_FP64M) {
-    rc = ensure_adj_scratch(b);
-    if (rc) return rc;
-  }
   int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, fp64, false, own) / nm * nm);
   cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
   if (b->rs.cap < cap || (fp64 && !b->rs.fp64) || (own != (b->rs.d_eVT != nullptr))) {
@@ -360,7 +359,7 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
     }
     float tp = 0.f;
     rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, fmt, &tp,
-                    fp64 ? kBuildP64 : kBuildMfma);
+                    precision == PCSF_FP64M ? kBuildP64M : fp64 ? kBuildP64 : kBuildMfma);
     if (rc) {
       char buf[64];
       std::snprintf(buf, sizeof buf, " (region block from %d)", r0);
@@ -371,8 +370,7 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
     if (c->timing) HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
     hipError_t e =
         (precision == PCSF_FP64M)
-            ? launch_forward_fp64m(m, b, ch, b->d_adj_scratch, adjoint_mfma_grid(m, true),
-                                   c->stream)
+            ? launch_forward_fp64m(m, b, ch, c->stream)
         : fp64 ? launch_prune_fp64_regions(m, b, ch, c->stream)
                : launch_prune_mfma_regions(m, b, precision, variant, ch, c->stream);
     if (e == hipErrorInvalidValue && precision == PCSF_FP64)
@@ -448,7 +446,7 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
       rc = ensure_adj_scratch(b);
       if (rc) return rc;
       e = launch_adjoint_mfma(m, b, ch, d_gbr ? d_gbr + (size_t)r0 * nm * m->nbr : nullptr,
-                              b->d_adj_scratch, adjoint_mfma_grid(m, false), c->stream);
+                              b->d_adj_scratch, adjoint_mfma_grid(m), c->stream);
     }
     HIP_TRY(c, e);
     HIP_TRY(c, launch_region_reduce_aligned(b, ch, b->d_gloglik, b->d_grad, c->stream,
@@ -1050,7 +1048,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   if (rc) return rc;
   if (precision == PCSF_FP64M && b->nreg > 0) {
     // region-aligned columns, the model's sets for every region
-    if ((rc = ensure_aligned(b)) || (rc = ensure_adj_scratch(b))) return rc;
+    if ((rc = ensure_aligned(b))) return rc;
   }
   b->t_pbuild = tp;
   b->last_rho.resize((size_t)b->nreg * m->nm);
@@ -1066,9 +1064,7 @@ int pcsf_forward(pcsf_batch* b, const double* rho, int rho_per_region, int preci
   all.nblocks = b->al.ld / kAlignCols;
   all.sets = &m->fixed;
   if (precision == PCSF_FP64M) {
-    e = b->nreg > 0 ? launch_forward_fp64m(m, b, all, b->d_adj_scratch, adjoint_mfma_grid(m, true),
-                                           c->stream)
-                    : hipSuccess;
+    e = b->nreg > 0 ? launch_forward_fp64m(m, b, all, c->stream) : hipSuccess;
   } else if (precision == PCSF_FP64) {
     e = launch_prune_fp64(m, b, c->stream);
     if (e == hipErrorInvalidValue)
@@ -1108,8 +1104,9 @@ int pcsf_forward_noalloc(pcsf_model* m, const uint8_t* d_codes, int64_t n_cols,
   if (m->tree_only) return set_err(c, PCSF_EINVAL, kTreeOnly);
   if (precision == PCSF_FP64M)
     return set_err(c, PCSF_EINVAL,
-                   "pcsf_forward_noalloc: PCSF_FP64M keeps one FP64 tile per tree node and "
-                   "column in device memory it allocates; use PCSF_FP64 here");
+                   "pcsf_forward_noalloc: PCSF_FP64M reads P in a layout of its own and the "
+                   "codes region-aligned, both in device memory it allocates; use PCSF_FP64 "
+                   "here");
   if (!known_precision(precision))
     return set_err(c, PCSF_EINVAL, "pcsf_forward_noalloc: unknown precision");
   if (n_cols < 0 || n_regions < 0 || (n_cols > 0 && !d_codes) || !d_region_offsets ||

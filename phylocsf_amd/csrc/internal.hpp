@@ -35,8 +35,11 @@ int frag_format(int precision);  // -1 for a precision without MFMA tiles
 struct PSets {
   int cap = 0;                 // sets allocated
   int fmt = -1;                // format of d_Pfrag as last built
-  bool fp64 = false;           // P64T allocated
+  bool fp64 = false;           // P64T and P64F allocated
   double* d_P64T = nullptr;    // cap * nbr * 4096 FP64 P^T (P64T[b][a] = P[a][b])
+  double* d_P64F = nullptr;    // cap * nbr * kTile64 FP64 P for the matrix-core forward:
+                               //   an internal branch's P in v_mfma_f64_16x16x4f64
+                               //   A-fragment order, a leaf branch's 65 leaf messages
   float* d_PT32 = nullptr;     // cap * nbr * 65 * 64 FP32 leaf messages: row c < 64 is
                                //   P[a][c] / P[c][c], row 64 all ones (`Marginalize)
   float* d_Pfrag = nullptr;    // cap * nbr * kTileFloats: off-diagonal of P in MFMA
@@ -179,7 +182,14 @@ hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uin
 // (P64T, if the sets have it), the MFMA kernels' leaf rows and tiles in
 // format fmt (PT32, Pfrag, lnPcc), the tangent's D (DT32, Dfrag, if the sets
 // have them).
-constexpr unsigned kBuildP64 = 1, kBuildMfma = 2, kBuildTangent = 4;
+constexpr unsigned kBuildP64 = 1, kBuildMfma = 2, kBuildTangent = 4, kBuildP64M = 8;
+// Doubles per branch of PSets::d_P64F.  Internal branch: P[a][b] at
+// ((a / 16) * 16 + b / 4) * 64 + a % 16 + 16 (b % 4) (the A-fragment order of
+// fp64_tiles.hpp).  Leaf branch: 65 messages of 64 entries, message c < 64
+// the column P[.][c] and message 64 all ones (`Marginalize), each stored so
+// that lane group g's 16 rows are contiguous: row 16 mo + 4 v + g at
+// c * 64 + 16 g + 4 mo + v.
+constexpr int kTile64 = 65 * 64;
 hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int fmt,
                          unsigned outputs, hipStream_t s);
 hipError_t launch_cherry_tables(const pcsf_model* m, const PSets& sets, int nsets,
@@ -219,14 +229,15 @@ struct AdjointArgs {
   double* grad_branch;       // (region - r0) x nm x nbr, added to (may be null)
   double* scratch;
 };
-int adjoint_mfma_grid(const pcsf_model* m, bool forward);
+int adjoint_mfma_grid(const pcsf_model* m);
 size_t adjoint_mfma_scratch_doubles(const pcsf_model* m, int grid);
 hipError_t launch_adjoint_mfma(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
                                double* grad_branch, double* scratch, int grid, hipStream_t s);
-// the same kernel's inside pass only: ln z and the root's expected ln pi per
-// column into the batch's aligned per-column buffers (precision PCSF_FP64M)
+// The FP64 forward on matrix cores (precision PCSF_FP64M, prune_fp64m.hip):
+// ln z and the root's expected ln pi per column of the chunk's region blocks
+// into the batch's aligned per-column buffers, from the sets' P64F.
 hipError_t launch_forward_fp64m(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
-                                double* scratch, int grid, hipStream_t s);
+                                hipStream_t s);
 // the omega test's rate model of every active set of a chunk from its twelve
 // settings (d_qs: nsets * 12): eigensystem and root prior into the sets'
 // own-model arrays

@@ -25,13 +25,8 @@
 // beta, and so needs no more memory.  Per-branch sums are reduced over the
 // wave's 16 columns, then over the workgroup in LDS, and added to the region's
 // total with one FP64 atomic per (workgroup, branch).  No rescaling, as the
-// reference (FP64 range).
-//
-// The same kernel without its outside pass (kFwd) is the FP64 forward on
-// matrix cores (precision PCSF_FP64M): per column ln z and the root
-// posterior's expected ln pi (PhyloLik.ml:127-138 at the root, the elpr_anc
-// term of PhyloCSF.ml), with the priors per model or per set (the omega
-// test's own models).
+// reference (FP64 range).  (The FP64 forward on matrix cores, which needs
+// no outside pass, keeps its tiles in registers: prune_fp64m.hip.)
 #include "fp64_tiles.hpp"
 
 namespace {
@@ -51,12 +46,9 @@ __device__ __forceinline__ Tile leaf_message(const double* __restrict__ PT, int 
 }
 
 
-template <bool kFwd>
 __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArgs p) {
-  __shared__ double Qf[kFwd ? 1 : 4096];  // Q of the task's model, A-fragment order
-  // the current P_c or P_c^T; the forward alternates two buffers, so that one
-  // barrier per node suffices
-  __shared__ double Af[kFwd ? 2 * 4096 : 4096];
+  __shared__ double Qf[4096];  // Q of the task's model, A-fragment order
+  __shared__ double Af[4096];  // the current P_c or P_c^T
   extern __shared__ double gsum[];        // nbr: per-branch sums of the task's workgroup
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, grp = lane >> 4;
   const int j = lane & 15;
@@ -77,22 +69,15 @@ __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArg
     const bool valid = col - p.astart[r] < p.off[r + 1] - p.off[r];
     const double* PT = p.P64T + (size_t)set * nbr * 4096;  // PT[br][b * 64 + a] = P[a][b]
     __syncthreads();  // the previous task is done with Qf, Af and gsum
-    if (!kFwd) {
-      if (q != staged_q) {
-        stage(Qf, p.QT + (size_t)q * 4096, false, tid);  // QT[k * 64 + a] = Q[a][k]
-        staged_q = q;
-      }
-      for (int br = tid; br < nbr; br += 64 * kWaves) gsum[br] = 0.0;
-      __syncthreads();  // Qf and gsum ready (a two-leaf tree has no barrier before its use)
+    if (q != staged_q) {
+      stage(Qf, p.QT + (size_t)q * 4096, false, tid);  // QT[k * 64 + a] = Q[a][k]
+      staged_q = q;
     }
+    for (int br = tid; br < nbr; br += 64 * kWaves) gsum[br] = 0.0;
+    __syncthreads();  // Qf and gsum ready (a two-leaf tree has no barrier before its use)
     // ---- inside pass (PhyloLik.ml:78-92), messages to the slots; nodes
-    // nl .. n - 2 take a product with their P, whose tile the forward
-    // fetches one node ahead (the outside pass's registers leave the
-    // adjoint no room for it)
+    // nl .. n - 2 take a product with their P
     double z = 0.0;
-    Share nxt;
-    if (kFwd && nl <= n - 2) fetch(nxt, PT + (size_t)nl * 4096, tid);
-    int buf = 0;
     for (int i = nl; i < n; ++i) {
       const int l = p.lc[i], rr = p.rc[i];
       Tile a = (l < nl) ? leaf_message(PT + (size_t)l * 4096, p.codes[(size_t)l * p.ld + col], grp)
@@ -110,38 +95,13 @@ __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArg
 #pragma unroll
           for (int v = 0; v < 4; ++v) pr.r[mo][v] = p.pi[pq * 64 + 16 * mo + 4 * v + grp];
         z = column_dot(a, pr);
-        if (kFwd) {  // sum_x (alpha_x pi_x / z) ln pi_x
-#pragma unroll
-          for (int mo = 0; mo < 4; ++mo)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              a.r[mo][v] *= pr.r[mo][v];
-              pr.r[mo][v] = p.lnpi[pq * 64 + 16 * mo + 4 * v + grp];
-            }
-          const double e = column_dot(a, pr);
-          if (grp == 0 && valid) {
-            p.logz[(size_t)q * p.ald + col] = log(z);
-            p.coldrho[(size_t)q * p.ald + col] = (z != 0.0) ? e / z : 0.0;
-          }
-        }
         break;
       }
-      double* A = Af + (kFwd ? buf * 4096 : 0);
-      if (kFwd) {
-        // buffer `buf` was last read two nodes ago, before the previous
-        // node's barrier
-        put(A, nxt, false, tid);
-        __syncthreads();
-        if (i + 1 <= n - 2) fetch(nxt, PT + (size_t)(i + 1) * 4096, tid);
-      } else {
-        __syncthreads();  // Af free
-        stage(A, PT + (size_t)i * 4096, false, tid);
-        __syncthreads();
-      }
-      store(slots + (size_t)(i - nl) * kTileDoubles, gemm(A, a, lane), lane);
-      buf ^= 1;
+      __syncthreads();  // Af free
+      stage(Af, PT + (size_t)i * 4096, false, tid);
+      __syncthreads();
+      store(slots + (size_t)(i - nl) * kTileDoubles, gemm(Af, a, lane), lane);
     }
-    if (kFwd) continue;
     const double zinv = (z != 0.0) ? 1.0 / z : 0.0;
     // ---- outside pass, parents before children
     double drho = 0.0;
@@ -214,13 +174,8 @@ size_t adjoint_mfma_scratch_doubles(const pcsf_model* m, int grid) {
   return (size_t)grid * kWaves * (m->n - m->nl) * kTileDoubles;
 }
 
-// resident workgroups per CU: two for both kernels (the outside pass: 243
-// registers and 64 KiB of LDS; the forward: 220 and two 32 KiB P buffers --
-// a single-buffered forward at three per CU measured no faster)
-int adjoint_mfma_grid(const pcsf_model* m, bool forward) {
-  (void)forward;
-  return m->ctx->num_cus * 2;
-}
+// resident workgroups per CU: two (243 registers, 64 KiB of LDS)
+int adjoint_mfma_grid(const pcsf_model* m) { return m->ctx->num_cus * 2; }
 
 namespace {
 
@@ -267,17 +222,7 @@ hipError_t launch_adjoint_mfma(const pcsf_model* m, const pcsf_batch* b, const R
   const int64_t ntask = ch.nblocks * m->nm;
   const int g = (int)std::min<int64_t>(ntask, grid);
   if (g <= 0) return hipSuccess;
-  hipLaunchKernelGGL(adjoint_mfma_kernel<false>, dim3(g), dim3(64 * kWaves),
+  hipLaunchKernelGGL(adjoint_mfma_kernel, dim3(g), dim3(64 * kWaves),
                      m->nbr * sizeof(double), s, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_forward_fp64m(const pcsf_model* m, const pcsf_batch* b, const RegionChunk& ch,
-                                double* scratch, int grid, hipStream_t s) {
-  const AdjointArgs a = make_args(m, b, ch, scratch);
-  const int64_t ntask = ch.nblocks * m->nm;
-  const int g = (int)std::min<int64_t>(ntask, grid);
-  if (g <= 0) return hipSuccess;
-  hipLaunchKernelGGL(adjoint_mfma_kernel<true>, dim3(g), dim3(64 * kWaves), 0, s, a);
   return hipGetLastError();
 }

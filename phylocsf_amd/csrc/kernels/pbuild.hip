@@ -12,7 +12,8 @@
 // one 64-bit atomicMin, so the record does not depend on which workgroup ran
 // first (fail_key below; all ones when nothing failed).
 //
-// Outputs, all [set][branch][...]; a leaf branch (br < nl) is read only
+// Outputs, all [set][branch][...] (which of them: kBuild* of launch_pbuild);
+// a leaf branch (br < nl) is read only
 // through its leaf messages and an internal one only through its tile, so
 // PT32 and lnPcc are written for leaf branches and Pfrag for internal ones:
 //   P64T  FP64 P transposed (P64T[b][a] = P[a][b]): coalesced GEMV operand of
@@ -28,6 +29,9 @@
 //         v_mfma_f32_16x16x32_{f16,bf16};
 //         followed by the 64 FP32 values P[a][a] - 1 (the diagonal term, which
 //         every precision adds in VALU arithmetic).
+//   P64F  FP64 P for the matrix-core forward (kTile64 doubles per branch):
+//         an internal branch's P in v_mfma_f64_16x16x4f64 A-fragment order,
+//         a leaf branch's 65 messages (columns of P, then all ones)
 //   DT32, Dfrag  (tangent sets only) D = dP(rho b)/drho = b Q P in the layouts
 //         of PT32 (divided by P[c][c]; row 64 zero) and of the FP32 tile
 //         (dense), for leaf and internal branches respectively.
@@ -97,7 +101,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     int nbw, int nl, double* __restrict__ P64T, float* __restrict__ PT32,
     float* __restrict__ Pfrag, int fmt, double* __restrict__ lnPcc,
     unsigned long long* __restrict__ perr, float* __restrict__ DT32,
-    float* __restrict__ Dfrag) {
+    float* __restrict__ Dfrag, double* __restrict__ P64F) {
   __shared__ double Bs[64 * 64];  // V^T, swizzled
   __shared__ double ptot[4][64], poff[4][64];  // per wave, per row of P
   __shared__ int pjbad[4][64];
@@ -237,6 +241,25 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     // A leaf branch is read only through its leaf messages, an internal branch
     // only through its tile: each writes just its own kind.
     const bool leaf = br < nl;
+    if (P64F) {
+      // FP64 P for the matrix-core forward (layout at kTile64): element r of
+      // tile ct is P[a][b], a = 16 ct + lr = 16 mo + 4 v + g (mo = ct,
+      // v = lr / 4, g = lr % 4), b = 16 w + 4 r + lk
+      double* out = P64F + sb * kTile64;
+      if (leaf) {
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            out[(16 * w + 4 * r + lk) * 64 + 16 * (lr & 3) + 4 * ct + (lr >> 2)] = acc[ct][r];
+        if (tid < 64) out[4096 + tid] = 1.0;
+      } else {
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) out[(ct * 16 + 4 * w + r) * 64 + lane] = acc[ct][r];
+      }
+    }
     if (leaf && PT32) {
       float* out = PT32 + sb * (kLeafRows * 64);
 #pragma unroll
@@ -378,6 +401,7 @@ hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int 
                          unsigned outputs, hipStream_t s) {
   if (nsets <= 0) return hipSuccess;
   const bool p64 = (outputs & kBuildP64) && sets.fp64;
+  const bool p64m = (outputs & kBuildP64M) && sets.fp64;
   const bool mfma = (outputs & kBuildMfma) != 0;
   const bool tangent = (outputs & kBuildTangent) && sets.d_Dfrag;
   // several branches per workgroup once there are enough matrices to fill
@@ -394,6 +418,6 @@ hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int 
                      nbw, m->nl,
                      p64 ? sets.d_P64T : nullptr, mfma ? sets.d_PT32 : nullptr,
                      mfma ? sets.d_Pfrag : nullptr, fmt, sets.d_lnPcc, m->d_perr, sets.d_DT32,
-                     sets.d_Dfrag);
+                     sets.d_Dfrag, p64m ? sets.d_P64F : nullptr);
   return hipGetLastError();
 }

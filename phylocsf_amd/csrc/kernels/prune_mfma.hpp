@@ -102,6 +102,10 @@ struct MfmaParams {
   // pi / lnpi hold one root prior per set instead of per model (sets with
   // rate models of their own)
   int prior_per_set;
+  // region mode over the model's own sets (set q in every region)
+  int shared_sets;
+  // the FP64 matrix-core forward only: P of the sets (PSets::d_P64F)
+  const double* P64F;
 };
 
 struct Task {
@@ -141,7 +145,7 @@ __device__ __forceinline__ bool next_task(const MfmaParams& p, int64_t& j, Task*
     }
     const int r = p.block_region[((p.g0 + t->g) * GC) / kAlignCols];
     if (r < 0) continue;
-    t->set = (r - p.r0) * p.nm + t->q;
+    t->set = p.shared_sets ? t->q : (r - p.r0) * p.nm + t->q;
     if (p.active[t->set]) return true;
   }
 }

@@ -7,6 +7,7 @@
 // and the launch shapes the autotuner most often picks; other requests fall
 // back to the interpreter.
 #pragma once
+#include "prune_fp64m.hpp"
 #include "prune_mfma.hpp"
 #include "prune_tangent.hpp"
 
@@ -17,7 +18,8 @@ typedef hipError_t (*TreeLaunch)(const MfmaParams&, int precision, int variant, 
 
 // One generated program: the tree's name, whether it reads cherry tables,
 // its op words (to match a model's program), its launcher and (programs
-// without tables) the launcher of its tangent pass.
+// without tables) the launchers of its tangent pass and FP64 matrix-core
+// forward.
 struct TreeProgram {
   const char* name;
   int tables;
@@ -25,6 +27,7 @@ struct TreeProgram {
   int n_words;
   TreeLaunch launch;
   TreeLaunch tangent;
+  TreeLaunch fp64m;
 };
 
 // Every generated program (host function of the generated registry).

@@ -1,6 +1,7 @@
 """GPU parity of the tree-specialised kernels (straight-line op programs
 generated at build time for the shipped trees) against the op-program
-interpreter and the FP64 path.
+interpreter and the FP64 path: the MFMA kernels at FP32 and split FP16, and
+the FP64 forward on matrix cores.
 
 Both run the same op code (kernels/prune_mfma.hpp); the specialised kernel
 only replaces the dispatch.  They agree to 1e-12 relative (the compiler may
@@ -61,6 +62,18 @@ def test_specialised_matches_interpreter(ctx, name):
                 assert np.allclose(a, i, rtol=1e-12, atol=1e-12), (name, prec, shape, k)
             assert np.all(np.abs(out[True][0] - ref) <= 1e-6 * np.maximum(1.0, np.abs(ref)))
             assert np.all(np.abs(out[True][1] - ref_r) <= 1e-6 * np.maximum(1.0, np.abs(ref_r)))
+    # the FP64 forward on matrix cores: the generated program against the
+    # interpreter and the plain FP64 kernel, to FP64 rounding
+    out = {}
+    for spec in (True, False):
+        m.specialized(spec)
+        fixed, _ = b.forward(1.0, P.PCSF_FP64M)
+        per, _ = b.forward(rho, P.PCSF_FP64M, per_region=True)
+        out[spec] = (fixed, per)
+    m.specialized(True)
+    for k, r in ((0, ref), (1, ref_r)):
+        assert np.allclose(out[True][k], out[False][k], rtol=1e-12, atol=1e-12), (name, k)
+        assert np.all(np.abs(out[True][k] - r) <= 1e-12 * np.maximum(1.0, np.abs(r))), (name, k)
 
 
 def test_species_subtree_uses_interpreter(ctx):
