@@ -40,9 +40,9 @@ hipError_t launch_forward_fp64m(const pcsf_model* m, const pcsf_batch* b, const 
     case 1:
     case 2: return launch_ns<2, Interp<2>>(prm, cus, s);
     case 3: return launch_ns<3, Interp<3>>(prm, cus, s);
-    case 4: return launch_ns<4, Interp<4>>(prm, cus, s);
-    case 5: return launch_ns<5, Interp<5>>(prm, cus, s);
-    case 6: return launch_ns<6, Interp<6>>(prm, cus, s);
+    case 4: return launch_ns<4, Interp<4>, 1>(prm, cus, s);
+    case 5: return launch_ns<5, Interp<5>, 1>(prm, cus, s);
+    case 6: return launch_ns<6, Interp<6>, 1>(prm, cus, s);
     default: return hipErrorInvalidValue;
   }
 }

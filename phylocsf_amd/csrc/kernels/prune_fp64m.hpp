@@ -147,8 +147,11 @@ struct Walk64 {
   }
 };
 
-template <int NS, class PROG>
-__global__ __launch_bounds__(kNT, 2) void prune_fp64m_kernel(MfmaParams p) {
+// MINB workgroups per CU: two (a generated program keeps only its live
+// tiles in registers), one for the interpreter above three live tiles (it
+// keeps every slot live through its dispatch; at two per CU it spills)
+template <int NS, class PROG, int MINB>
+__global__ __launch_bounds__(kNT, MINB) void prune_fp64m_kernel(MfmaParams p) {
   __shared__ __attribute__((aligned(16))) double Pbuf[2][4096];
   extern __shared__ __attribute__((aligned(16))) uint8_t codes_s[];  // nl x kGC
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
@@ -200,9 +203,9 @@ __global__ __launch_bounds__(kNT, 2) void prune_fp64m_kernel(MfmaParams p) {
   }
 }
 
-template <int NS, class PROG>
+template <int NS, class PROG, int MINB = 2>
 hipError_t launch_ns(const MfmaParams& prm, int cus, hipStream_t s) {
-  auto kern = prune_fp64m_kernel<NS, PROG>;
+  auto kern = prune_fp64m_kernel<NS, PROG, MINB>;
   const size_t dyn = ((size_t)prm.nl * kGC + 15) / 16 * 16;
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kNT, dyn);
