@@ -219,7 +219,7 @@ def main():
         log("fp32 MFMA: %.3f ms/step, %.3e col/s" % (fp32["ms_per_step"], fp32["value"]))
 
     # ---- plain FP64 GPU kernel on the same batch (the 50x denominator)
-    fp64_rate, dmax = None, None
+    fp64_rate, dmax, fp64m = None, None, None
     if args.fp64_steps > 0:
         batch.forward(1.0, P.PCSF_FP64)
         ctx.synchronize()
@@ -235,6 +235,19 @@ def main():
         if fp32 is not None:
             fp32["max_abs_delta_score_db_vs_fp64"] = float(
                 np.max(np.abs(P.DB * (llf[:, 0] - llf[:, 1]) - s64)))
+        # the FP64 forward on matrix cores (the CLI's default precision)
+        llm, _ = batch.forward(1.0, P.PCSF_FP64M)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        nm_steps = max(2, args.fp64_steps)
+        for _ in range(nm_steps):
+            batch.forward(1.0, P.PCSF_FP64M)
+        ctx.synchronize()
+        fp64m = {"value": ncols * nm_steps / (time.perf_counter() - t0),
+                 "unit": "codon-columns/s",
+                 "max_abs_delta_score_db_vs_fp64": float(
+                     np.max(np.abs(P.DB * (llm[:, 0] - llm[:, 1]) - s64))) if len(s64) else 0.0}
+        log("FP64 matrix cores: %.3e col/s" % fp64m["value"])
 
     # ---- roofline of the dominant kernel (fused pruning: MFMA-bound; its HBM
     # traffic is the codes plus per-column results, ~nl + 32 B per column)
@@ -295,6 +308,7 @@ def main():
         },
         "plain_fp64_gpu": {"value": fp64_rate, "unit": "codon-columns/s",
                            "speedup_of_value": value / fp64_rate if fp64_rate else None},
+        "fp64_matrix_cores": fp64m,
         "accuracy": {"max_abs_delta_score_db_vs_fp64": dmax, "bound_db": 1e-3},
         "fp32": fp32,
         "roofline": roofline,
