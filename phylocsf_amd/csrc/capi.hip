@@ -79,6 +79,7 @@ hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0,
   chk(dalloc(&ps.d_active, (size_t)cap));
   if (fp64) chk(dalloc(&ps.d_P64T, n * 4096));
   if (fp64) chk(dalloc(&ps.d_P64F, n * kTile64));
+  if (fp64) chk(dalloc(&ps.d_P64FT, n * 4096));
   if (tangent) {
     chk(dalloc(&ps.d_DT32, n * kLeafRows * 64));
     chk(dalloc(&ps.d_Dfrag, n * 4096));
@@ -107,6 +108,7 @@ hipError_t alloc_psets(PSets& ps, int cap, int nbr, bool fp64, int nctab = 0,
 void free_psets(PSets& ps) {
   dfree(ps.d_P64T);
   dfree(ps.d_P64F);
+  dfree(ps.d_P64FT);
   dfree(ps.d_PT32);
   dfree(ps.d_Pfrag);
   dfree(ps.d_lnPcc);
@@ -169,7 +171,8 @@ int pbuild_failure(pcsf_model* m, unsigned long long key, const double* rho_host
 // from the same P(t) built in FP64 by Q.Diag.real_to_Pt's rules.
 int run_pbuild(pcsf_model* m, PSets& ps, int nsets, const double* rho_host, int fmt,
                float* ms_out,
-               unsigned outputs = kBuildP64 | kBuildP64M | kBuildMfma | kBuildTangent) {
+               unsigned outputs = kBuildP64 | kBuildP64M | kBuildP64MT | kBuildMfma |
+                                  kBuildTangent) {
   pcsf_ctx* c = m->ctx;
   HIP_TRY(c, hipMemsetAsync(m->d_perr, 0xff, sizeof(unsigned long long), c->stream));
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -272,7 +275,7 @@ int ensure_aligned(pcsf_batch* b) {
 // Bytes of one transition set on the device.
 size_t set_bytes(const pcsf_model* m, bool fp64, bool tangent = false, bool own_model = false) {
   return (size_t)m->nbr * (kLeafRows * 64 * 4 + kTileFloats * 4 + kLeafRows * 8 +
-                           (fp64 ? (4096 + kTile64) * 8 : 0) +
+                           (fp64 ? (2 * 4096 + kTile64) * 8 : 0) +
                            (tangent ? (kLeafRows + 64) * 64 * 4 : 0)) +
          (own_model ? (2 * 4096 + 3 * 64) * 8 : 0);
 }
@@ -437,7 +440,7 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
                               c->stream));
     float tp = 0.f;
     rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, kFragF32, &tp,
-                    tangent ? (kBuildMfma | kBuildTangent) : kBuildP64);
+                    tangent ? (kBuildMfma | kBuildTangent) : (kBuildP64M | kBuildP64MT));
     if (rc) return rc;
     hipError_t e;
     if (tangent) {

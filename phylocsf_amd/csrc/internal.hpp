@@ -40,6 +40,8 @@ struct PSets {
   double* d_P64F = nullptr;    // cap * nbr * kTile64 FP64 P for the matrix-core forward:
                                //   an internal branch's P in v_mfma_f64_16x16x4f64
                                //   A-fragment order, a leaf branch's 65 leaf messages
+  double* d_P64FT = nullptr;   // cap * nbr * 4096 FP64 P^T in that A-fragment order (the
+                               //   outside pass's products; internal branches)
   float* d_PT32 = nullptr;     // cap * nbr * 65 * 64 FP32 leaf messages: row c < 64 is
                                //   P[a][c] / P[c][c], row 64 all ones (`Marginalize)
   float* d_Pfrag = nullptr;    // cap * nbr * kTileFloats: off-diagonal of P in MFMA
@@ -182,7 +184,8 @@ hipError_t launch_transpose_codes(const uint8_t* src, int64_t ncols, int nl, uin
 // (P64T, if the sets have it), the MFMA kernels' leaf rows and tiles in
 // format fmt (PT32, Pfrag, lnPcc), the tangent's D (DT32, Dfrag, if the sets
 // have them).
-constexpr unsigned kBuildP64 = 1, kBuildMfma = 2, kBuildTangent = 4, kBuildP64M = 8;
+constexpr unsigned kBuildP64 = 1, kBuildMfma = 2, kBuildTangent = 4, kBuildP64M = 8,
+                   kBuildP64MT = 16;
 // Doubles per branch of PSets::d_P64F.  Internal branch: P[a][b] at
 // ((a / 16) * 16 + b / 4) * 64 + a % 16 + 16 (b % 4) (the A-fragment order of
 // fp64_tiles.hpp).  Leaf branch: 65 messages of 64 entries, message c < 64
@@ -212,7 +215,8 @@ struct AdjointArgs {
   int64_t c0, nblocks;       // first column and block count of the chunk
   int n, nl, nm, nbr;
   const int32_t *lc, *rc;
-  const double* P64T;        // sets: P^T per branch (P64T[b * 64 + a] = P[a][b])
+  const double* P64F;        // sets: P per branch (PSets::d_P64F layout)
+  const double* P64FT;       // sets: P^T per branch in A-fragment order
   const double* QT;          // per model: Q^T
   const double* pi;          // per model, or per set with prior_per_set
   const double* lnpi;        // its logarithm

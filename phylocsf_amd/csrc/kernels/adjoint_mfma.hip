@@ -33,30 +33,52 @@ namespace {
 
 using namespace fp64t;
 
-// the message of leaf c: column `code` of P_c (PT[code * 64 + row] =
-// P[row][code]), or ones for `Marginalize
-__device__ __forceinline__ Tile leaf_message(const double* __restrict__ PT, int code, int grp) {
+// the message of leaf branch `br` for `code`: column `code` of P, or ones
+// for `Marginalize (message 64), from the branch's leaf messages in P64F
+// (row 16 mo + 4 v + g at code * 64 + 16 g + 4 mo + v)
+__device__ __forceinline__ Tile leaf_message(const double* __restrict__ PF, int code, int grp) {
+  const int c = (code == PCSF_MARGINALIZE) ? 64 : code;
+  const double2* src = reinterpret_cast<const double2*>(PF + c * 64 + 16 * grp);
   Tile t;
 #pragma unroll
   for (int mo = 0; mo < 4; ++mo)
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-      t.r[mo][v] = (code == PCSF_MARGINALIZE) ? 1.0 : PT[code * 64 + 16 * mo + 4 * v + grp];
+    for (int h = 0; h < 2; ++h) {
+      const double2 d = src[2 * mo + h];
+      t.r[mo][2 * h] = d.x;
+      t.r[mo][2 * h + 1] = d.y;
+    }
   return t;
 }
 
+typedef __attribute__((address_space(3))) double L3;
+typedef __attribute__((address_space(3))) void* LdsPtr;
+
+// A 4096-double tile stored in A-fragment order copied straight from memory
+// into LDS (16 bytes per lane at the wave's base + 16 lane, the order the tile
+// has in memory): no registers, no LDS stores; the barrier before its first
+// read waits for it.
+__device__ __forceinline__ void copy_tile(L3* dst, const double* src, int wave, int tid) {
+  typedef const __attribute__((address_space(1))) void* GPtr;
+#pragma unroll
+  for (int k = 0; k < 4096 / 2 / (64 * kWaves); ++k)
+    __builtin_amdgcn_global_load_lds((GPtr)(src + (tid + 64 * kWaves * k) * 2),
+                                     (LdsPtr)(dst + (64 * wave + 64 * kWaves * k) * 2), 16, 0, 0);
+}
 
 __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArgs p) {
-  __shared__ double Qf[4096];  // Q of the task's model, A-fragment order
-  __shared__ double Af[4096];  // the current P_c or P_c^T
+  // the inside pass alternates its P tiles between the two buffers; the
+  // outside pass holds Q in Qf and each P^T in Af
+  __shared__ __attribute__((aligned(16))) double Qf[4096];
+  __shared__ __attribute__((aligned(16))) double Af[4096];
   extern __shared__ double gsum[];        // nbr: per-branch sums of the task's workgroup
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, grp = lane >> 4;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63,
+            grp = lane >> 4;
   const int j = lane & 15;
   const int n = p.n, nl = p.nl, nm = p.nm, nbr = p.nbr;
   // this wave's message / beta slots, one tile per internal node
   double* slots = p.scratch + ((size_t)blockIdx.x * kWaves + wave) * (n - nl) * kTileDoubles;
   const int64_t ntask = p.nblocks * nm;
-  int staged_q = -1;
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
     const int64_t blk = task / nm;
     const int q = (int)(task - blk * nm);
@@ -67,24 +89,22 @@ __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArg
     if (!p.active[set]) continue;
     const int64_t col = b0 + wave * 16 + j;
     const bool valid = col - p.astart[r] < p.off[r + 1] - p.off[r];
-    const double* PT = p.P64T + (size_t)set * nbr * 4096;  // PT[br][b * 64 + a] = P[a][b]
+    const double* PF = p.P64F + (size_t)set * nbr * kTile64;    // PF[br * kTile64 + ...]
+    const double* PFT = p.P64FT + (size_t)set * nbr * 4096;     // PFT[br * 4096 + ...]
+    auto msg = [&](int c) -> Tile {
+      return (c < nl) ? leaf_message(PF + (size_t)c * kTile64, p.codes[(size_t)c * p.ld + col], grp)
+                      : load(slots + (size_t)(c - nl) * kTileDoubles, lane);
+    };
     __syncthreads();  // the previous task is done with Qf, Af and gsum
-    if (q != staged_q) {
-      stage(Qf, p.QT + (size_t)q * 4096, false, tid);  // QT[k * 64 + a] = Q[a][k]
-      staged_q = q;
-    }
     for (int br = tid; br < nbr; br += 64 * kWaves) gsum[br] = 0.0;
-    __syncthreads();  // Qf and gsum ready (a two-leaf tree has no barrier before its use)
     // ---- inside pass (PhyloLik.ml:78-92), messages to the slots; nodes
-    // nl .. n - 2 take a product with their P
+    // nl .. n - 2 take a product with their P, each tile copied into the
+    // idle buffer one node ahead
+    if (nl <= n - 2) copy_tile((L3*)Af, PF + (size_t)nl * kTile64, wave, tid);
     double z = 0.0;
     for (int i = nl; i < n; ++i) {
-      const int l = p.lc[i], rr = p.rc[i];
-      Tile a = (l < nl) ? leaf_message(PT + (size_t)l * 4096, p.codes[(size_t)l * p.ld + col], grp)
-                        : load(slots + (size_t)(l - nl) * kTileDoubles, lane);
-      const Tile mr = (rr < nl)
-                          ? leaf_message(PT + (size_t)rr * 4096, p.codes[(size_t)rr * p.ld + col], grp)
-                          : load(slots + (size_t)(rr - nl) * kTileDoubles, lane);
+      Tile a = msg(p.lc[i]);
+      const Tile mr = msg(p.rc[i]);
 #pragma unroll
       for (int mo = 0; mo < 4; ++mo) a.r[mo] *= mr.r[mo];  // alpha_i
       if (i == n - 1) {
@@ -97,22 +117,23 @@ __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArg
         z = column_dot(a, pr);
         break;
       }
-      __syncthreads();  // Af free
-      stage(Af, PT + (size_t)i * 4096, false, tid);
-      __syncthreads();
-      store(slots + (size_t)(i - nl) * kTileDoubles, gemm(Af, a, lane), lane);
+      const bool odd = (i - nl) & 1;  // tile i in Qf, else in Af
+      __syncthreads();  // tile i landed; the other buffer was last read a node ago
+      if (i + 1 <= n - 2) copy_tile((L3*)(odd ? Af : Qf), PF + (size_t)(i + 1) * kTile64, wave, tid);
+      store(slots + (size_t)(i - nl) * kTileDoubles, gemm(odd ? Qf : Af, a, lane), lane);
     }
+    __syncthreads();  // the inside pass is done with both buffers (and gsum is zero)
+    stage(Qf, p.QT + (size_t)q * 4096, false, tid);  // QT[k * 64 + a] = Q[a][k]
+    __syncthreads();
     const double zinv = (z != 0.0) ? 1.0 / z : 0.0;
-    // ---- outside pass, parents before children
+    // ---- outside pass, parents before children; each P^T is copied into
+    // Af while the child's Q product runs
     double drho = 0.0;
     for (int pn = n - 1; pn >= nl; --pn) {
       const int kid[2] = {p.lc[pn], p.rc[pn]};
       Tile m[2];
 #pragma unroll
-      for (int s = 0; s < 2; ++s)
-        m[s] = (kid[s] < nl) ? leaf_message(PT + (size_t)kid[s] * 4096,
-                                            p.codes[(size_t)kid[s] * p.ld + col], grp)
-                             : load(slots + (size_t)(kid[s] - nl) * kTileDoubles, lane);
+      for (int s = 0; s < 2; ++s) m[s] = msg(kid[s]);
       Tile bp;
       if (pn == n - 1) {
 #pragma unroll
@@ -132,6 +153,10 @@ __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArg
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         const int c = kid[s];
+        if (c >= nl) {
+          __syncthreads();  // Af no longer read
+          copy_tile((L3*)Af, PFT + (size_t)c * 4096, wave, tid);
+        }
         // d ln z / d t_c = w_c . (Q m_c) / z for this column.  The product
         // runs on every lane: an MFMA takes its A rows from all 64 lanes, so
         // a lane of a padding column that skipped the fragment loads would
@@ -148,9 +173,7 @@ __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArg
           if (lane == 0) atomicAdd(&gsum[c], t);
         }
         if (c >= nl) {  // beta_c = P_c^T w_c into c's slot
-          __syncthreads();
-          stage(Af, PT + (size_t)c * 4096, true, tid);  // P^T[b][a] = PT[b * 64 + a]
-          __syncthreads();
+          __syncthreads();  // P_c^T landed
           store(slots + (size_t)(c - nl) * kTileDoubles, gemm(Af, w[s], lane), lane);
         }
       }
@@ -193,7 +216,8 @@ AdjointArgs make_args(const pcsf_model* m, const pcsf_batch* b, const RegionChun
   a.nbr = m->nbr;
   a.lc = m->d_lc;
   a.rc = m->d_rc;
-  a.P64T = ch.sets->d_P64T;
+  a.P64F = ch.sets->d_P64F;
+  a.P64FT = ch.sets->d_P64FT;
   a.QT = m->d_QT;
   a.prior_per_set = ch.sets->d_epi ? 1 : 0;
   a.pi = a.prior_per_set ? ch.sets->d_epi : m->d_pi;

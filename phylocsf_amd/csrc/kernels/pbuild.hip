@@ -32,6 +32,8 @@
 //   P64F  FP64 P for the matrix-core forward (kTile64 doubles per branch):
 //         an internal branch's P in v_mfma_f64_16x16x4f64 A-fragment order,
 //         a leaf branch's 65 messages (columns of P, then all ones)
+//   P64FT FP64 P^T of internal branches in that A-fragment order (the outside
+//         pass's products)
 //   DT32, Dfrag  (tangent sets only) D = dP(rho b)/drho = b Q P in the layouts
 //         of PT32 (divided by P[c][c]; row 64 zero) and of the FP32 tile
 //         (dense), for leaf and internal branches respectively.
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
     int nbw, int nl, double* __restrict__ P64T, float* __restrict__ PT32,
     float* __restrict__ Pfrag, int fmt, double* __restrict__ lnPcc,
     unsigned long long* __restrict__ perr, float* __restrict__ DT32,
-    float* __restrict__ Dfrag, double* __restrict__ P64F) {
+    float* __restrict__ Dfrag, double* __restrict__ P64F, double* __restrict__ P64FT) {
   __shared__ double Bs[64 * 64];  // V^T, swizzled
   __shared__ double ptot[4][64], poff[4][64];  // per wave, per row of P
   __shared__ int pjbad[4][64];
@@ -260,6 +262,16 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
           for (int r = 0; r < 4; ++r) out[(ct * 16 + 4 * w + r) * 64 + lane] = acc[ct][r];
       }
     }
+    if (P64FT && !leaf) {
+      // P^T in the same order: P^T[b][a] = P[a][b] at
+      // ((b / 16) * 16 + a / 4) * 64 + b % 16 + 16 (a % 4)
+      double* out = P64FT + sb * 4096;
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          out[(16 * w + 4 * ct + (lr >> 2)) * 64 + 4 * r + lk + 16 * (lr & 3)] = acc[ct][r];
+    }
     if (leaf && PT32) {
       float* out = PT32 + sb * (kLeafRows * 64);
 #pragma unroll
@@ -402,6 +414,7 @@ hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int 
   if (nsets <= 0) return hipSuccess;
   const bool p64 = (outputs & kBuildP64) && sets.fp64;
   const bool p64m = (outputs & kBuildP64M) && sets.fp64;
+  const bool p64mt = (outputs & kBuildP64MT) && sets.fp64;
   const bool mfma = (outputs & kBuildMfma) != 0;
   const bool tangent = (outputs & kBuildTangent) && sets.d_Dfrag;
   // several branches per workgroup once there are enough matrices to fill
@@ -418,6 +431,7 @@ hipError_t launch_pbuild(const pcsf_model* m, const PSets& sets, int nsets, int 
                      nbw, m->nl,
                      p64 ? sets.d_P64T : nullptr, mfma ? sets.d_PT32 : nullptr,
                      mfma ? sets.d_Pfrag : nullptr, fmt, sets.d_lnPcc, m->d_perr, sets.d_DT32,
-                     sets.d_Dfrag, p64m ? sets.d_P64F : nullptr);
+                     sets.d_Dfrag, p64m ? sets.d_P64F : nullptr,
+                     p64mt ? sets.d_P64FT : nullptr);
   return hipGetLastError();
 }
