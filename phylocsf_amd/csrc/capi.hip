@@ -2073,9 +2073,11 @@ int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
       for (int b = 0; b < 64; ++b) t += P[((size_t)br * 64 + a) * 64 + b];
       rowsum[(size_t)br * 64 + a] = t;
     }
-  int32_t *d_lc = nullptr, *d_rc = nullptr, *d_leaves = nullptr;
+  const std::vector<int32_t> seq = estep_sequence(n, lc, rc);
+  int32_t *d_lc = nullptr, *d_rc = nullptr, *d_leaves = nullptr, *d_seq = nullptr;
   double *d_P = nullptr, *d_rs = nullptr, *d_prior = nullptr, *d_w = nullptr, *d_z = nullptr,
-         *d_U = nullptr, *d_V = nullptr, *d_ec = nullptr, *d_scr = nullptr;
+         *d_U = nullptr, *d_V = nullptr, *d_ec = nullptr, *d_scr = nullptr, *d_Pf = nullptr,
+         *d_PTf = nullptr;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) {
     if (e == hipSuccess) e = x;
@@ -2083,6 +2085,9 @@ int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
   chk(dalloc(&d_lc, (size_t)n));
   chk(dalloc(&d_rc, (size_t)n));
   chk(dalloc(&d_P, (size_t)nbr * 4096));
+  chk(dalloc(&d_Pf, (size_t)nbr * 4096));
+  chk(dalloc(&d_PTf, (size_t)nbr * 4096));
+  chk(dalloc(&d_seq, std::max<size_t>(1, seq.size())));
   chk(dalloc(&d_rs, (size_t)nbr * 64));
   chk(dalloc(&d_prior, 64));
   chk(dalloc(&d_leaves, (size_t)chunk * nl));
@@ -2101,6 +2106,8 @@ int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
   up(d_lc, lc.data(), (size_t)n * 4);
   up(d_rc, rc.data(), (size_t)n * 4);
   up(d_P, P, (size_t)nbr * 4096 * 8);
+  up(d_seq, seq.data(), seq.size() * 4);
+  if (e == hipSuccess) e = launch_estep_frags(d_P, nbr, d_Pf, d_PTf, c->stream);
   up(d_rs, rowsum.data(), rowsum.size() * 8);
   up(d_prior, prior, 64 * 8);
   up(d_ec, ecounts, (size_t)nbr * 4096 * 8);  // add_branch_posteriors adds
@@ -2114,6 +2121,10 @@ int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
     a.lc = d_lc;
     a.rc = d_rc;
     a.P = d_P;
+    a.Pf = d_Pf;
+    a.PTf = d_PTf;
+    a.seq = d_seq;
+    a.nseq = (int)seq.size();
     a.rowsum = d_rs;
     a.prior = d_prior;
     a.leaves = d_leaves;
@@ -2129,11 +2140,11 @@ int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
     down(z + c0, d_z, (size_t)nc * 8);
   }
   down(ecounts, d_ec, (size_t)nbr * 4096 * 8);
-  for (auto* q : {d_lc, d_rc, d_leaves}) {
+  for (auto* q : {d_lc, d_rc, d_leaves, d_seq}) {
     int32_t* x = q;
     dfree(x);
   }
-  for (auto* q : {d_P, d_rs, d_prior, d_w, d_z, d_U, d_V, d_ec, d_scr}) {
+  for (auto* q : {d_P, d_rs, d_prior, d_w, d_z, d_U, d_V, d_ec, d_scr, d_Pf, d_PTf}) {
     double* x = q;
     dfree(x);
   }

@@ -298,6 +298,11 @@ struct EstepArgs {
   int n = 0, nl = 0;
   const int32_t *lc = nullptr, *rc = nullptr;
   const double* P = nullptr;        // (n-1) x 64 x 64, row-major per branch
+  const double* Pf = nullptr;       // (n-1) x 4096: P in A-fragment order (fp64_tiles.hpp)
+  const double* PTf = nullptr;      // (n-1) x 4096: P^T in A-fragment order
+  const int32_t* seq = nullptr;     // the passes' products in order: branch b for P_b,
+                                    //   -1 - b for P_b^T (estep_sequence)
+  int nseq = 0;
   const double* rowsum = nullptr;   // (n-1) x 64: P's row sums (`Marginalize leaf messages)
   const double* prior = nullptr;    // 64
   const int32_t* leaves = nullptr;  // ncols x nl: state, -1 `Marginalize, >= 64 zeros
@@ -311,5 +316,10 @@ struct EstepArgs {
 };
 int estep_grid(const pcsf_ctx* c);
 size_t estep_scratch_doubles(int n, int grid);
+// the products of estep_passes_kernel in the order it runs them
+std::vector<int32_t> estep_sequence(int n, const std::vector<int32_t>& lc,
+                                    const std::vector<int32_t>& rc);
+// Pf and PTf of EstepArgs from P (nbr row-major 64 x 64 matrices)
+hipError_t launch_estep_frags(const double* P, int nbr, double* Pf, double* PTf, hipStream_t s);
 hipError_t launch_estep(const EstepArgs& a, const double* P, double* ecounts, int grid,
                         hipStream_t s);

@@ -51,21 +51,6 @@ __device__ __forceinline__ Tile leaf_message(const double* __restrict__ PF, int 
   return t;
 }
 
-typedef __attribute__((address_space(3))) double L3;
-typedef __attribute__((address_space(3))) void* LdsPtr;
-
-// A 4096-double tile stored in A-fragment order copied straight from memory
-// into LDS (16 bytes per lane at the wave's base + 16 lane, the order the tile
-// has in memory): no registers, no LDS stores; the barrier before its first
-// read waits for it.
-__device__ __forceinline__ void copy_tile(L3* dst, const double* src, int wave, int tid) {
-  typedef const __attribute__((address_space(1))) void* GPtr;
-#pragma unroll
-  for (int k = 0; k < 4096 / 2 / (64 * kWaves); ++k)
-    __builtin_amdgcn_global_load_lds((GPtr)(src + (tid + 64 * kWaves * k) * 2),
-                                     (LdsPtr)(dst + (64 * wave + 64 * kWaves * k) * 2), 16, 0, 0);
-}
-
 __global__ __launch_bounds__(64 * kWaves, 2) void adjoint_mfma_kernel(AdjointArgs p) {
   // the inside pass alternates its P tiles between the two buffers; the
   // outside pass holds Q in Qf and each P^T in Af

@@ -65,13 +65,26 @@ __device__ __forceinline__ void put_column(double* __restrict__ dst, const Tile&
 }
 
 __global__ __launch_bounds__(64 * kWaves, 2) void estep_passes_kernel(EstepArgs p) {
-  __shared__ double Af[4096];  // the current P_c or P_c^T in A-fragment order
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, grp = lane >> 4;
+  // P_c or P_c^T in A-fragment order, double-buffered: product k of a block
+  // (p.seq[k]) reads buffer k & 1 while product k + 1's tile is copied into
+  // the other
+  __shared__ __attribute__((aligned(16))) double Af[2][4096];
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63,
+            grp = lane >> 4;
   const int j = lane & 15;
   const int n = p.n, nl = p.nl;
   double* slots = p.scratch + ((size_t)blockIdx.x * kWaves + wave) * (n - nl) * kTileDoubles;
   const int64_t nblocks = (p.ncols + kCols - 1) / kCols;
+  auto issue = [&](int k) {  // product k's tile into buffer k & 1
+    if (k >= p.nseq) return;
+    const int code = p.seq[k];
+    const double* src = (code >= 0) ? p.Pf + (size_t)code * 4096 : p.PTf + (size_t)(-1 - code) * 4096;
+    copy_tile((L3*)&Af[k & 1][0], src, wave, tid);
+  };
   for (int64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+    __syncthreads();  // the previous block is done with both buffers
+    issue(0);
+    int kp = 0;  // products done in this block
     const int64_t col = blk * kCols + wave * 16 + j;
     const bool valid = col < p.ncols;
     const int64_t ccol = valid ? col : p.ncols - 1;  // a readable column for padding lanes
@@ -97,10 +110,10 @@ __global__ __launch_bounds__(64 * kWaves, 2) void estep_passes_kernel(EstepArgs 
         z = column_dot(a, pr);
         break;
       }
-      __syncthreads();  // Af free
-      stage(Af, p.P + (size_t)i * 4096, true, tid);  // M = P_i
-      __syncthreads();
-      store(slots + (size_t)(i - nl) * kTileDoubles, gemm(Af, a, lane), lane);
+      __syncthreads();  // P_i landed; the other buffer is no longer read
+      issue(kp + 1);
+      store(slots + (size_t)(i - nl) * kTileDoubles, gemm(Af[kp & 1], a, lane), lane);
+      ++kp;
     }
     if (valid && grp == 0) p.z[col] = z;
     const double w = valid ? (p.weights ? p.weights[col] : 1.0) : 0.0;
@@ -149,10 +162,10 @@ __global__ __launch_bounds__(64 * kWaves, 2) void estep_passes_kernel(EstepArgs 
             for (int mo = 0; mo < 4; ++mo) lv.r[mo] = (f64x4){0.0, 0.0, 0.0, 0.0};
           put_column(p.V + o, lv, grp);
         } else {  // beta_c = P_c^T (beta_p o m_s) into c's slot
-          __syncthreads();
-          stage(Af, p.P + (size_t)c * 4096, false, tid);  // M = P_c^T
-          __syncthreads();
-          store(slots + (size_t)(c - nl) * kTileDoubles, gemm(Af, wt[s], lane), lane);
+          __syncthreads();  // P_c^T landed; the other buffer is no longer read
+          issue(kp + 1);
+          store(slots + (size_t)(c - nl) * kTileDoubles, gemm(Af[kp & 1], wt[s], lane), lane);
+          ++kp;
         }
       }
     }
@@ -197,7 +210,39 @@ __global__ __launch_bounds__(256) void estep_counts_kernel(const double* __restr
     }
 }
 
+// P and P^T of every branch into the A-fragment order of fp64_tiles.hpp:
+// M[row][col] at ((row / 16) * 16 + col / 4) * 64 + row % 16 + 16 (col % 4)
+__global__ __launch_bounds__(256) void estep_frags_kernel(const double* __restrict__ P,
+                                                          double* __restrict__ Pf,
+                                                          double* __restrict__ PTf) {
+  const int br = blockIdx.x;
+  const double* src = P + (size_t)br * 4096;
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int row = e >> 6, col = e & 63;
+    const int f = ((row >> 4) * 16 + (col >> 2)) * 64 + (row & 15) + 16 * (col & 3);
+    Pf[(size_t)br * 4096 + f] = src[row * 64 + col];   // M = P
+    PTf[(size_t)br * 4096 + f] = src[col * 64 + row];  // M = P^T
+  }
+}
+
 }  // namespace
+
+hipError_t launch_estep_frags(const double* P, int nbr, double* Pf, double* PTf, hipStream_t s) {
+  if (nbr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(estep_frags_kernel, dim3(nbr), dim3(256), 0, s, P, Pf, PTf);
+  return hipGetLastError();
+}
+
+std::vector<int32_t> estep_sequence(int n, const std::vector<int32_t>& lc,
+                                    const std::vector<int32_t>& rc) {
+  const int nl = (n + 1) / 2;
+  std::vector<int32_t> seq;
+  for (int i = nl; i < n - 1; ++i) seq.push_back(i);  // inside: P_i, nodes in order
+  for (int pn = n - 1; pn >= nl; --pn)                // outside: P_c^T, internal children
+    for (int c : {lc[pn], rc[pn]})
+      if (c >= nl) seq.push_back(-1 - c);
+  return seq;
+}
 
 int estep_grid(const pcsf_ctx* c) { return c->num_cus * 2; }
 

@@ -61,6 +61,21 @@ __device__ __forceinline__ void stage(double* __restrict__ frag, const double* _
   }
 }
 
+typedef __attribute__((address_space(3))) double L3;
+typedef __attribute__((address_space(3))) void* LdsPtr;
+
+// A 4096-double tile already in A-fragment order in memory, copied straight
+// into LDS (16 bytes per lane at the wave's base + 16 lane, the order the tile
+// has in memory): no registers, no LDS stores.  A barrier before its first
+// read waits for it (its release fence waits for the copy).
+__device__ __forceinline__ void copy_tile(L3* dst, const double* src, int wave, int tid) {
+  typedef const __attribute__((address_space(1))) void* GPtr;
+#pragma unroll
+  for (int k = 0; k < 4096 / 2 / (64 * kWaves); ++k)
+    __builtin_amdgcn_global_load_lds((GPtr)(src + (tid + 64 * kWaves * k) * 2),
+                                     (LdsPtr)(dst + (64 * wave + 64 * kWaves * k) * 2), 16, 0, 0);
+}
+
 // acc = M B for the staged M and this wave's tile B
 __device__ __forceinline__ Tile gemm(const double* __restrict__ frag, const Tile& b, int lane) {
   Tile acc;
