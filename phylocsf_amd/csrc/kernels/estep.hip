@@ -12,7 +12,9 @@
 //   for the branch into child c of p with sibling s,
 //     u_c = w beta_p o m_s / z   (zero where beta_p <= 0 or z <= 0, the
 //                                 entries the reference skips)
-//     v_c = alpha_c              (the leaf vector for a leaf)
+//     v_c = alpha_c              (internal c; a leaf's v is its leaf
+//                                 vector, rebuilt from the leaf's state by
+//                                 the counts kernel)
 //     beta_c = P_c^T (beta_p o m_s),
 //   u and v written per (branch, column) for the next kernel, z per column.
 // estep_counts_kernel: per branch, S = sum_col u v^T as a 64 x 64 product
@@ -39,19 +41,6 @@ __device__ __forceinline__ Tile leaf_message(const double* __restrict__ Pc,
       t.r[mo][v] = (code >= 0 && code < 64) ? Pc[row * 64 + code]
                    : (code < 0)             ? rowsum[row]
                                             : 0.0;
-    }
-  return t;
-}
-
-// the leaf vector itself: e_code, ones, or zeros
-__device__ __forceinline__ Tile leaf_vector(int code, int grp) {
-  Tile t;
-#pragma unroll
-  for (int mo = 0; mo < 4; ++mo)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = 16 * mo + 4 * v + grp;
-      t.r[mo][v] = (code < 0) ? 1.0 : (code == row) ? 1.0 : 0.0;
     }
   return t;
 }
@@ -155,13 +144,9 @@ __global__ __launch_bounds__(64 * kWaves, 2) void estep_passes_kernel(EstepArgs 
           for (int v = 0; v < 4; ++v)
             u.r[mo][v] = (bp.r[mo][v] > 0.0) ? wt[s].r[mo][v] * scale : 0.0;
         put_column(p.U + o, u, grp);
-        if (c < nl) {
-          Tile lv = leaf_vector(codes[c], grp);
-          if (!valid)
-#pragma unroll
-            for (int mo = 0; mo < 4; ++mo) lv.r[mo] = (f64x4){0.0, 0.0, 0.0, 0.0};
-          put_column(p.V + o, lv, grp);
-        } else {  // beta_c = P_c^T (beta_p o m_s) into c's slot
+        if (c >= nl) {  // beta_c = P_c^T (beta_p o m_s) into c's slot
+          // (a leaf branch's v is its leaf vector, which the counts kernel
+          // rebuilds from the codes)
           __syncthreads();  // P_c^T landed; the other buffer is no longer read
           issue(kp + 1);
           store(slots + (size_t)(c - nl) * kTileDoubles, gemm(Af[kp & 1], wt[s], lane), lane);
@@ -173,13 +158,18 @@ __global__ __launch_bounds__(64 * kWaves, 2) void estep_passes_kernel(EstepArgs 
 }
 
 // ecounts_br += P_br o sum_col u v^T over columns [0, ncols) (ncols a
-// multiple of 4; U and V hold ldu columns per branch).  Workgroup
+// multiple of 4; U and V hold ldu columns per branch; for a leaf branch
+// br < nl, v is the leaf vector of the column's state in leaves: one-hot,
+// ones for `Marginalize, zeros for a state >= 64 or a column past nvalid).
+// Workgroup
 // (branch, split) takes the split's share of the columns -- wave w the rows
 // 16 w .. 16 w + 15 and all 64 columns (four 16 x 16 accumulators) -- and
 // adds P o its partial sum with FP64 atomics (P o S is linear in S).
 __global__ __launch_bounds__(256) void estep_counts_kernel(const double* __restrict__ U,
                                                            const double* __restrict__ V,
                                                            int64_t ldu, int64_t ncols,
+                                                           const int32_t* __restrict__ leaves,
+                                                           int nl, int64_t nvalid,
                                                            const double* __restrict__ P,
                                                            double* __restrict__ ecounts) {
   const int br = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -191,12 +181,24 @@ __global__ __launch_bounds__(256) void estep_counts_kernel(const double* __restr
   f64x4 acc[4];
 #pragma unroll
   for (int no = 0; no < 4; ++no) acc[no] = (f64x4){0.0, 0.0, 0.0, 0.0};
-  for (int64_t j0 = jb; j0 < je; j0 += 4) {
-    const double a = Ub[(j0 + kk) * 64 + 16 * w + i];  // A[16 w + i][j0 + kk]
+  if (br < nl) {  // (uniform over the workgroup)
+    for (int64_t j0 = jb; j0 < je; j0 += 4) {
+      const double a = Ub[(j0 + kk) * 64 + 16 * w + i];  // A[16 w + i][j0 + kk]
+      const int code = (j0 + kk < nvalid) ? leaves[(size_t)(j0 + kk) * nl + br] : 64;
 #pragma unroll
-    for (int no = 0; no < 4; ++no)
-      acc[no] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Vb[(j0 + kk) * 64 + 16 * no + i], acc[no],
-                                                     0, 0, 0);
+      for (int no = 0; no < 4; ++no) {
+        const double v = (code < 0 || code == 16 * no + i) ? 1.0 : 0.0;
+        acc[no] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, v, acc[no], 0, 0, 0);
+      }
+    }
+  } else {
+    for (int64_t j0 = jb; j0 < je; j0 += 4) {
+      const double a = Ub[(j0 + kk) * 64 + 16 * w + i];  // A[16 w + i][j0 + kk]
+#pragma unroll
+      for (int no = 0; no < 4; ++no)
+        acc[no] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Vb[(j0 + kk) * 64 + 16 * no + i],
+                                                       acc[no], 0, 0, 0);
+    }
   }
   // acc[no][v] of lane i + 16 g: S[16 w + 4 v + g][16 no + i]
   const double* Pb = P + (size_t)br * 4096;
@@ -264,6 +266,6 @@ hipError_t launch_estep(const EstepArgs& a, const double* P, double* ecounts, in
   const int splits = (int)std::max<int64_t>(
       1, std::min<int64_t>({16, nblocks, (4 * grid + a.n - 2) / (a.n - 1)}));
   hipLaunchKernelGGL(estep_counts_kernel, dim3(a.n - 1, splits), dim3(256), 0, s, a.U, a.V,
-                     a.ldu, nblocks * kCols, P, ecounts);
+                     a.ldu, nblocks * kCols, a.leaves, a.nl, a.ncols, P, ecounts);
   return hipGetLastError();
 }
