@@ -272,6 +272,20 @@ int ensure_aligned(pcsf_batch* b) {
   return PCSF_OK;
 }
 
+// Device memory for the per-region transition sets of a batch: regions go
+// through in chunks of as many sets as fit, 24 GiB (a fraction of the 288 GB
+// of one MI355X, so that several large batches can live side by side) unless
+// PHYLOCSF_SET_BUDGET_MB says otherwise (read on every pass; small values
+// force many chunks, which the tests use).
+size_t set_budget() {
+  const char* e = std::getenv("PHYLOCSF_SET_BUDGET_MB");
+  if (e && *e) {
+    const long long mb = std::atoll(e);
+    if (mb > 0) return (size_t)mb << 20;
+  }
+  return (size_t)24 << 30;
+}
+
 // Bytes of one transition set on the device.
 size_t set_bytes(const pcsf_model* m, bool fp64, bool tangent = false, bool own_model = false) {
   return (size_t)m->nbr * (kLeafRows * 64 * 4 + kTileFloats * 4 + kLeafRows * 8 +
@@ -314,13 +328,18 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
   if (rc) return rc;
   const bool fp64 = fp64_precision(precision);
   const int fmt = fp64 ? kFragF32 : frag_format(precision);
-  const size_t budget = (size_t)8 << 30;
   const bool own = sm != nullptr;
-  int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, fp64, false, own) / nm * nm);
+  // a reallocated pool keeps what the old one served (FP64 and tangent
+  // data), so that maximisers alternating passes do not rebuild it each time
+  const bool keep = own == (b->rs.d_eVT != nullptr);
+  const bool want_fp64 = fp64 || (keep && b->rs.fp64);
+  const bool want_tangent = keep && b->rs.d_Dfrag != nullptr;
+  int cap = (int)std::max<size_t>(
+      nm, set_budget() / set_bytes(m, want_fp64, want_tangent, own) / nm * nm);
   cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
-  if (b->rs.cap < cap || (fp64 && !b->rs.fp64) || (own != (b->rs.d_eVT != nullptr))) {
+  if (b->rs.cap < cap || (fp64 && !b->rs.fp64) || !keep) {
     free_psets(b->rs);
-    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, fp64, 0, false, own));
+    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, want_fp64, 0, want_tangent, own));
   }
   const int chunk_regions = b->rs.cap / nm;
   int variant = fp64 ? 0 : m->mfma_variant[precision];
@@ -408,12 +427,17 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
   const int nm = m->nm;
   int rc = ensure_aligned(b);
   if (rc) return rc;
-  const size_t budget = (size_t)8 << 30;
-  int cap = (int)std::max<size_t>(nm, budget / set_bytes(m, !tangent, tangent) / nm * nm);
+  // (sets with rate models of their own are not kept: the gradient passes
+  // run on the models' eigensystems)
+  const bool keep = b->rs.d_eVT == nullptr;
+  const bool want_fp64 = !tangent || (keep && b->rs.fp64);
+  const bool want_tangent = tangent || (keep && b->rs.d_Dfrag != nullptr);
+  int cap = (int)std::max<size_t>(
+      nm, set_budget() / set_bytes(m, want_fp64, want_tangent) / nm * nm);
   cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
-  if (b->rs.cap < cap || (!tangent && !b->rs.fp64) || (tangent && !b->rs.d_Dfrag)) {
+  if (b->rs.cap < cap || (!tangent && !b->rs.fp64) || (tangent && !b->rs.d_Dfrag) || !keep) {
     free_psets(b->rs);
-    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, !tangent, 0, tangent));
+    HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, want_fp64, 0, want_tangent));
   }
   const int chunk_regions = b->rs.cap / nm;
   std::vector<uint8_t> act;

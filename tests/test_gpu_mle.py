@@ -192,3 +192,45 @@ def test_mle_find_init_retry_path_matches_oracle(ctx, params_dir):
             lhi, _ = om.lpr(q, 10.0, codes[off[r]:off[r + 1]])
             retried += int(l1 <= llo or l1 <= lhi)
     assert retried >= 6  # the retry loop ran for most (region, model) pairs
+
+
+@pytest.mark.parametrize("paramset", ["12flies", "100vertebrates"])
+def test_region_chunks_agree_with_one_chunk(ctx, paramset, monkeypatch):
+    """Per-region passes go through the regions in chunks of as many
+    transition sets as fit the device budget (PHYLOCSF_SET_BUDGET_MB): with
+    a 1 MiB budget every region is a chunk of its own, and the per-region
+    forwards (FP32, FP64, FP64 on matrix cores) and both maximisers give what
+    one chunk gives.  Columns and regions are computed independently of the
+    chunking (one workgroup per region in the reduction), so the tolerances
+    only allow for nothing."""
+    _say(f"region chunks, {paramset}")
+    m = P.PhyloCSFModel.from_paramset(ctx, paramset)
+    rng = np.random.default_rng(8)
+    R = 23
+    lengths = rng.integers(1, 150, R)
+    lengths[3] = 0
+    codes, off = m.simulate(lengths, rng.integers(0, 2, R), rng.uniform(0.3, 3.0, R),
+                            gap_p=0.1, seed=12)
+    rho = rng.uniform(0.05, 6.0, (R, 2))
+
+    def run():
+        b = P.Batch(m, codes, off)
+        out = {}
+        for prec in (P.PCSF_FP32, P.PCSF_FP64, P.PCSF_FP64M):
+            out[prec] = b.forward(rho, prec, per_region=True)
+        out["grad"] = P.score(b, "mle_grad", P.PCSF_FP16S)
+        out["brent"] = P.score(b, "mle", P.PCSF_FP64M)
+        b.close()
+        return out
+
+    monkeypatch.delenv("PHYLOCSF_SET_BUDGET_MB", raising=False)
+    one = run()
+    monkeypatch.setenv("PHYLOCSF_SET_BUDGET_MB", "1")
+    many = run()
+    for prec in (P.PCSF_FP32, P.PCSF_FP64, P.PCSF_FP64M):
+        for a, c in zip(one[prec], many[prec]):
+            np.testing.assert_allclose(c, a, rtol=1e-13, atol=0, err_msg=str(prec))
+    for k in ("grad", "brent"):
+        np.testing.assert_array_equal(many[k]["status"], one[k]["status"])
+        np.testing.assert_allclose(many[k]["score"], one[k]["score"], rtol=0, atol=1e-9)
+        np.testing.assert_allclose(many[k]["rho"], one[k]["rho"], rtol=1e-9, atol=0)
