@@ -24,8 +24,7 @@ constexpr int kLeafRows = 65;  // leaf messages per branch: codons 0..63, 64 = `
 //              lo = RN((x - hi) x 2^13), in v_mfma_f32_16x16x32_f16 A order
 //              (4096 halves each) + diagonal
 //   kFragBF16  the same in BF16, unscaled, residual unscaled
-// The 64 FP32 diagonal entries follow at float 4096: P[a][a] - 1 in kFragF32,
-// P[a][a] in the half formats.
+// The 64 FP32 values P[a][a] - 1 follow at float 4096 in every format.
 constexpr int kFragF32 = 0, kFragF16 = 1, kFragBF16 = 2;
 // 2^kHalfExp: the scale of the FP16 P tiles and of their residuals.  2^13
 // keeps |x| < 2^13 and |lo| <= 2^14 inside FP16's range and their entries
@@ -46,7 +45,7 @@ struct PSets {
   float* d_PT32 = nullptr;     // cap * nbr * 65 * 64 FP32 leaf messages: row c < 64 is
                                //   P[a][c] / P[c][c], row 64 all ones (`Marginalize)
   float* d_Pfrag = nullptr;    // cap * nbr * kTileFloats: off-diagonal of P in MFMA
-                               //   A-fragment order, then the 64 diagonal entries
+                               //   A-fragment order, then the 64 values P[a][a] - 1
   double* d_lnPcc = nullptr;   // cap * nbr * 65 FP64 ln P[c][c] (0 for row 64)
   // tangent data (allocated on request): D = dP/drho = b Q P(rho b)
   float* d_DT32 = nullptr;     // cap * nbr * 65 * 64: row c < 64 D[a][c] / P[c][c], row 64 0

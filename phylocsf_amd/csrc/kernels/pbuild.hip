@@ -27,9 +27,8 @@
 //         zero) for v_mfma_f32_16x16x4f32, or hi and lo halves of the same
 //         off-diagonal (FP16 scaled by 2^kHalfExp, or BF16) for
 //         v_mfma_f32_16x16x32_{f16,bf16};
-//         followed by the 64 FP32 diagonal entries (the diagonal term, which
-//         every precision adds in VALU arithmetic): P[a][a] - 1 for the FP32
-//         tile, P[a][a] for the halves.
+//         followed by the 64 FP32 values P[a][a] - 1 (the diagonal term, which
+//         every precision adds in VALU arithmetic).
 //   P64F  FP64 P for the matrix-core forward (kTile64 doubles per branch):
 //         an internal branch's P in v_mfma_f64_16x16x4f64 A-fragment order,
 //         a leaf branch's 65 messages (columns of P, then all ones)
@@ -335,11 +334,7 @@ __global__ __launch_bounds__(256, WITH_D ? 2 : 3) void pbuild_kernel(
             }
           }
       }
-      // the diagonal: P_aa - 1 for the FP32 tile (its products add it to the
-      // partial in round-to-nearest arithmetic), P_aa itself for the half
-      // formats (one FMA per element; rounding P_aa to FP32 is far below the
-      // halves' 2^-22)
-      if (tid < 64) out[4096 + tid] = (float)(fmt == kFragF32 ? pdiag[tid] - 1.0 : pdiag[tid]);
+      if (tid < 64) out[4096 + tid] = (float)(pdiag[tid] - 1.0);
     }
     if constexpr (WITH_D) {
       // D = dP(rho b)/drho = b Q P(t) for the tangent kernel, in the layout

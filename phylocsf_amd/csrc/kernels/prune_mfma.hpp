@@ -22,9 +22,8 @@
 //    MI355X: about -2e-6 per column and model when the accumulator carries
 //    alpha).  So in every precision the MFMA only accumulates the
 //    off-diagonal mass of P from a zero accumulator (nonnegative terms, no
-//    cancellation), and m = alpha + fma(P_aa - 1, alpha, offdiag) (FP32;
-//    the half precisions: fma(P_aa, alpha, offdiag)) is finished in
-//    round-to-nearest VALU arithmetic.
+//    cancellation), and m = alpha + fma(P_aa - 1, alpha, offdiag) is
+//    finished in round-to-nearest VALU arithmetic.
 //  * Leaf children are gathers of a column of P normalised by its diagonal
 //    entry (so the dominant entry is exactly 1; ln P[c][c] is summed per
 //    column in FP64), or 1 for `Marginalize; cherries need no GEMM.  A lane
@@ -301,8 +300,8 @@ __device__ __forceinline__ void split_f16x8(const f32x4& a, const f32x4& b, f16x
 // the cross terms hi*lo + lo*hi, which carry 2^kHalfExp, into accx, both from
 // zero (the matrix cores' FP32 accumulation rounds toward zero, which would
 // bias s if it were the initial value); then the block's message
-// m = fma(P_aa, s, acc + 2^-13 accx) in round-to-nearest VALU arithmetic
-// (two FMAs per element), which runs while the next block's MFMAs do.  Columns whose
+// m = s + fma(P_aa - 1, s, acc + 2^-13 accx) in round-to-nearest VALU
+// arithmetic, which runs while the next block's MFMAs do.  Columns whose
 // partial is a constant vector (mask k) keep it.  Tile layout of the halves
 // (pbuild.hip): hi fragments, then lo fragments, 512 x 8 values each, in the
 // A-operand order of v_mfma_f32_16x16x32_{f16,bf16} with K permuted so that
@@ -351,7 +350,7 @@ __device__ __forceinline__ void gemm_half(TileG<G>& s, const MaskG<G>& k, const 
       ah[st] = Ah[f];
       if constexpr (PREC == kPrecF16S) al[st] = Al[f];
     }
-    const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);  // P_aa
+    const f32x4 dd = *reinterpret_cast<const f32x4*>(Ab + 4096 + 16 * mo + 4 * grp);
 #pragma unroll
     for (int c = 0; c < G; ++c) {
       f32x4 acc = (f32x4)(0.0f), accx = (f32x4)(0.0f);
@@ -367,7 +366,7 @@ __device__ __forceinline__ void gemm_half(TileG<G>& s, const MaskG<G>& k, const 
       for (int v = 0; v < 4; ++v) {
         float u = acc[v];
         if constexpr (PREC == kPrecF16S) u = fmaf(accx[v], kHalfUnscale, u);
-        m.g[c].r[mo][v] = fmaf(dd[v], s.g[c].r[mo][v], u);
+        m.g[c].r[mo][v] = s.g[c].r[mo][v] + fmaf(dd[v], s.g[c].r[mo][v], u);
       }
     }
   }
