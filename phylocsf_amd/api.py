@@ -454,7 +454,9 @@ class PhyloLik:
 
     @staticmethod
     def _inputs(tree, pms, prior, leaves, leaf_dist):
-        """The checks of PhyloLik.prepare; -> (P, prior, leaves, leaf_dist, children)."""
+        """The shape checks of PhyloLik.prepare (the library checks the leaf
+        states, `Distribution leaves without leaf_dist included, in one pass);
+        -> (P, prior, leaves, leaf_dist, children)."""
         k = len(prior)
         leaves = np.ascontiguousarray(leaves, dtype=np.int32)
         if leaves.ndim == 1:
@@ -463,8 +465,6 @@ class PhyloLik:
             raise InvalidArgument(-1, "CamlPaml.Infer.prepare: length(leaves) != leaves(t)")
         if pms.shape[0] < tree.n - 1:
             raise InvalidArgument(-1, "CamlPaml.Infer.prepare: not enough P matrices")
-        if (leaves == -2).any() and leaf_dist is None:
-            raise InvalidArgument(-1, "CamlPaml.Infer.prepare: `Distribution leaves need leaf_dist")
         P = np.ascontiguousarray(pms[:tree.n - 1], dtype=np.float64)
         pr = np.ascontiguousarray(prior, dtype=np.float64)
         ld = None if leaf_dist is None else np.ascontiguousarray(leaf_dist, dtype=np.float64)

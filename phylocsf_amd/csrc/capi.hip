@@ -562,6 +562,19 @@ int pcsf_ctx_create(int device, pcsf_ctx** out) {
 
 void pcsf_ctx_destroy(pcsf_ctx* c) {
   if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  EstepCache& es = c->es;
+  if (es.copy) (void)hipStreamSynchronize(es.copy);
+  dfree(es.U);
+  dfree(es.V);
+  dfree(es.scr);
+  for (int i = 0; i < 2; ++i) {
+    dfree(es.leaves[i]);
+    if (es.staged[i]) (void)hipHostFree(es.staged[i]);
+    if (es.uploaded[i]) (void)hipEventDestroy(es.uploaded[i]);
+    if (es.consumed[i]) (void)hipEventDestroy(es.consumed[i]);
+  }
+  if (es.copy) (void)hipStreamDestroy(es.copy);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -2082,6 +2095,66 @@ namespace {
 // cores (kernels/estep.hip): columns in chunks of up to 32,768, u and v of
 // every (branch, column) of a chunk in device memory, the counts added per
 // chunk.
+// The context's E-step buffers for chunks of `chunk` columns (grown, never
+// shrunk; a failed growth leaves them released).
+hipError_t estep_cache(pcsf_ctx* c, int n, int64_t ldu, int64_t chunk, int grid) {
+  EstepCache& es = c->es;
+  const int nl = (n + 1) / 2, nbr = n - 1;
+  hipError_t e = hipSuccess;
+  if (!es.copy) e = hipStreamCreateWithFlags(&es.copy, hipStreamNonBlocking);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+    if (!es.uploaded[i]) e = hipEventCreateWithFlags(&es.uploaded[i], hipEventDisableTiming);
+    if (e == hipSuccess && !es.consumed[i])
+      e = hipEventCreateWithFlags(&es.consumed[i], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) return e;
+  const size_t uv = (size_t)nbr * ldu * 64;
+  if (es.cap_uv < uv) {
+    dfree(es.U);
+    dfree(es.V);
+    es.cap_uv = 0;
+    e = dalloc(&es.U, uv);
+    if (e == hipSuccess) e = dalloc(&es.V, uv);
+    if (e != hipSuccess) {
+      dfree(es.U);
+      dfree(es.V);
+      return e;
+    }
+    es.cap_uv = uv;
+  }
+  const size_t scr = estep_scratch_doubles(n, grid);
+  if (es.cap_scr < scr) {
+    dfree(es.scr);
+    es.cap_scr = 0;
+    if ((e = dalloc(&es.scr, scr)) != hipSuccess) return e;
+    es.cap_scr = scr;
+  }
+  const size_t lv = (size_t)chunk * nl;
+  if (es.cap_leaves < lv) {
+    for (int i = 0; i < 2; ++i) {
+      dfree(es.leaves[i]);
+      if (es.staged[i]) (void)hipHostFree(es.staged[i]);
+      es.staged[i] = nullptr;
+    }
+    es.cap_leaves = 0;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+      e = dalloc(&es.leaves[i], lv);
+      if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&es.staged[i]), lv * 4, hipHostMallocDefault);
+    }
+    if (e != hipSuccess) {
+      for (int i = 0; i < 2; ++i) {
+        dfree(es.leaves[i]);
+        if (es.staged[i]) (void)hipHostFree(es.staged[i]);
+        es.staged[i] = nullptr;
+      }
+      return e;
+    }
+    es.cap_leaves = lv;
+  }
+  return hipSuccess;
+}
+
 int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
                const std::vector<int32_t>& rc, const double* P, const double* prior,
                const int32_t* leaves, int64_t ncols, const double* weights, double* z,
@@ -2098,14 +2171,15 @@ int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
       rowsum[(size_t)br * 64 + a] = t;
     }
   const std::vector<int32_t> seq = estep_sequence(n, lc, rc);
-  int32_t *d_lc = nullptr, *d_rc = nullptr, *d_leaves = nullptr, *d_seq = nullptr;
+  int32_t *d_lc = nullptr, *d_rc = nullptr, *d_seq = nullptr;
   double *d_P = nullptr, *d_rs = nullptr, *d_prior = nullptr, *d_w = nullptr, *d_z = nullptr,
-         *d_U = nullptr, *d_V = nullptr, *d_ec = nullptr, *d_scr = nullptr, *d_Pf = nullptr,
-         *d_PTf = nullptr;
+         *d_ec = nullptr, *d_Pf = nullptr, *d_PTf = nullptr;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) {
     if (e == hipSuccess) e = x;
   };
+  chk(estep_cache(c, n, ldu, chunk, grid));
+  EstepCache& es = c->es;
   chk(dalloc(&d_lc, (size_t)n));
   chk(dalloc(&d_rc, (size_t)n));
   chk(dalloc(&d_P, (size_t)nbr * 4096));
@@ -2114,13 +2188,9 @@ int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
   chk(dalloc(&d_seq, std::max<size_t>(1, seq.size())));
   chk(dalloc(&d_rs, (size_t)nbr * 64));
   chk(dalloc(&d_prior, 64));
-  chk(dalloc(&d_leaves, (size_t)chunk * nl));
-  if (weights) chk(dalloc(&d_w, (size_t)chunk));
-  chk(dalloc(&d_z, (size_t)chunk));
-  chk(dalloc(&d_U, (size_t)nbr * ldu * 64));
-  chk(dalloc(&d_V, (size_t)nbr * ldu * 64));
+  if (weights) chk(dalloc(&d_w, (size_t)ncols));
+  chk(dalloc(&d_z, (size_t)ncols));
   chk(dalloc(&d_ec, (size_t)nbr * 4096));
-  chk(dalloc(&d_scr, estep_scratch_doubles(n, grid)));
   auto up = [&](void* dst, const void* src, size_t bytes) {
     if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
   };
@@ -2135,10 +2205,30 @@ int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
   up(d_rs, rowsum.data(), rowsum.size() * 8);
   up(d_prior, prior, 64 * 8);
   up(d_ec, ecounts, (size_t)nbr * 4096 * 8);  // add_branch_posteriors adds
-  for (int64_t c0 = 0; c0 < ncols && e == hipSuccess; c0 += chunk) {
-    const int64_t nc = std::min(chunk, ncols - c0);
-    up(d_leaves, leaves + c0 * nl, (size_t)nc * nl * 4);
-    if (weights) up(d_w, weights + c0, (size_t)nc * 8);
+  if (weights) up(d_w, weights, (size_t)ncols * 8);
+  // Chunk k's leaf states go host -> pinned slot k % 2 -> device slot k % 2 on
+  // the copy stream while chunk k - 1 computes; a slot is refilled once the
+  // chunk two back has been copied out of it (host) and computed (device).
+  const int64_t nchunks = (ncols + chunk - 1) / chunk;
+  auto stage = [&](int64_t k) {
+    const int sl = (int)(k & 1);
+    const int64_t c0 = k * chunk, nc = std::min(chunk, ncols - c0);
+    if (k >= 2) {
+      chk(hipEventSynchronize(es.uploaded[sl]));
+      chk(hipStreamWaitEvent(es.copy, es.consumed[sl], 0));
+    }
+    if (e != hipSuccess) return;
+    std::memcpy(es.staged[sl], leaves + c0 * nl, (size_t)nc * nl * 4);
+    chk(hipMemcpyAsync(es.leaves[sl], es.staged[sl], (size_t)nc * nl * 4,
+                       hipMemcpyHostToDevice, es.copy));
+    chk(hipEventRecord(es.uploaded[sl], es.copy));
+  };
+  if (nchunks > 0) stage(0);
+  for (int64_t k = 0; k < nchunks && e == hipSuccess; ++k) {
+    if (k + 1 < nchunks) stage(k + 1);
+    const int sl = (int)(k & 1);
+    const int64_t c0 = k * chunk, nc = std::min(chunk, ncols - c0);
+    chk(hipStreamWaitEvent(c->stream, es.uploaded[sl], 0));
     EstepArgs a;
     a.n = n;
     a.nl = nl;
@@ -2151,24 +2241,29 @@ int estep_mfma(pcsf_ctx* c, int n, const std::vector<int32_t>& lc,
     a.nseq = (int)seq.size();
     a.rowsum = d_rs;
     a.prior = d_prior;
-    a.leaves = d_leaves;
-    a.weights = d_w;
+    a.leaves = es.leaves[sl];
+    a.weights = d_w ? d_w + c0 : nullptr;
     a.ncols = nc;
-    a.z = d_z;
-    a.U = d_U;
-    a.V = d_V;
+    a.z = d_z + c0;
+    a.U = es.U;
+    a.V = es.V;
     a.ldu = ldu;
-    a.scratch = d_scr;
+    a.scratch = es.scr;
     if (e == hipSuccess) e = launch_estep(a, d_P, d_ec, grid, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    down(z + c0, d_z, (size_t)nc * 8);
+    chk(hipEventRecord(es.consumed[sl], c->stream));
   }
+  // both streams drain before any buffer of this call is released
+  const hipError_t s1 = hipStreamSynchronize(c->stream);
+  const hipError_t s2 = es.copy ? hipStreamSynchronize(es.copy) : hipSuccess;
+  chk(s1);
+  chk(s2);
+  down(z, d_z, (size_t)ncols * 8);
   down(ecounts, d_ec, (size_t)nbr * 4096 * 8);
-  for (auto* q : {d_lc, d_rc, d_leaves, d_seq}) {
+  for (auto* q : {d_lc, d_rc, d_seq}) {
     int32_t* x = q;
     dfree(x);
   }
-  for (auto* q : {d_P, d_rs, d_prior, d_w, d_z, d_U, d_V, d_ec, d_scr, d_Pf, d_PTf}) {
+  for (auto* q : {d_P, d_rs, d_prior, d_w, d_z, d_ec, d_Pf, d_PTf}) {
     double* x = q;
     dfree(x);
   }
@@ -2195,14 +2290,17 @@ int lik_impl(pcsf_ctx* c, int k, int n, const int32_t* children, const double* P
     sib[a] = b;
     sib[b] = a;
   }
-  bool dist = false;
+  // a state >= k is a leaf vector of zeros, as raw_leaf makes it
+  // (PhyloLik.ml:11); below -2 is no leaf kind.  One branch-free pass (it
+  // vectorises) over what can be a few hundred MB of states.
+  int32_t vmin = 0, ndist = 0;
   for (int64_t e = 0; e < ncols * nl; ++e) {
-    // a state >= k is a leaf vector of zeros, as raw_leaf makes it
-    // (PhyloLik.ml:11); below -2 is no leaf kind
-    const int v = leaves[e];
-    if (v < -2) return set_err(c, PCSF_EINVAL, "CamlPaml.Infer.prepare: leaf state out of range");
-    dist = dist || v == -2;
+    const int32_t v = leaves[e];
+    vmin = std::min(vmin, v);
+    ndist |= (int32_t)(v == -2);
   }
+  if (vmin < -2) return set_err(c, PCSF_EINVAL, "CamlPaml.Infer.prepare: leaf state out of range");
+  const bool dist = ndist != 0;
   if (dist && !leaf_dist)
     return set_err(c, PCSF_EINVAL, "CamlPaml.Infer.prepare: `Distribution leaves need leaf_dist");
   if (ncols == 0) return PCSF_OK;

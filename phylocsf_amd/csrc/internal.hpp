@@ -66,12 +66,27 @@ struct PSets {
   double* d_elnpi = nullptr;   // cap * 64 its logarithm
 };
 
+// Device and pinned buffers of the E-step (capi.hip estep_mfma), kept by the
+// context between calls and grown as needed: the u, v hand-over of a chunk
+// (gigabytes), the passes' node tiles, and two leaf-state slots that a second
+// stream fills (from pinned staging) while the previous chunk computes.
+struct EstepCache {
+  double *U = nullptr, *V = nullptr, *scr = nullptr;
+  size_t cap_uv = 0, cap_scr = 0;  // doubles of U (= of V) and of scr
+  int32_t* leaves[2] = {nullptr, nullptr};
+  int32_t* staged[2] = {nullptr, nullptr};  // pinned host
+  size_t cap_leaves = 0;                    // int32 per slot
+  hipStream_t copy = nullptr;               // default priority, like the context's
+  hipEvent_t uploaded[2] = {nullptr, nullptr}, consumed[2] = {nullptr, nullptr};
+};
+
 struct pcsf_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
   bool timing = false;
   int num_cus = 256;
+  EstepCache es;
 };
 
 struct pcsf_model {
