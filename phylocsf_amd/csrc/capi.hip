@@ -339,12 +339,21 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
   cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
   if (b->rs.cap < cap || (fp64 && !b->rs.fp64) || !keep) {
     free_psets(b->rs);
+    b->eig_ok.clear();
+    b->eig_ref_ok.clear();
     HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, want_fp64, 0, want_tangent, own));
   }
   const int chunk_regions = b->rs.cap / nm;
+  // one chunk: set k is pair k on every pass, so a set's eigensystem can
+  // seed (or stand for) the next one's
+  const bool one_chunk = chunk_regions >= b->nreg;
+  if (!one_chunk) {
+    b->eig_ok.clear();
+    b->eig_ref_ok.clear();
+  }
   int variant = fp64 ? 0 : m->mfma_variant[precision];
   if (16 * kMfmaVariants[variant].wpg * kMfmaVariants[variant].g > kAlignCols) variant = 1;
-  std::vector<uint8_t> act;
+  std::vector<uint8_t> act, eigmode;
   b->t_pbuild = b->t_prune = b->t_reduce = 0.f;
   for (int32_t r0 = 0; r0 < b->nreg; r0 += chunk_regions) {
     const int32_t nr = std::min<int32_t>(chunk_regions, b->nreg - r0);
@@ -371,13 +380,65 @@ int forward_regions(pcsf_batch* b, const double* rho, const uint8_t* active, int
       const size_t k0 = (size_t)r0 * nm, nk = (size_t)nr * nm;
       if (b->setqs_cap < (size_t)b->rs.cap * 12) {
         dfree(b->d_setqs);
+        dfree(b->d_eigmode);
+        dfree(b->d_eigref);
         b->setqs_cap = 0;
+        b->eig_ref_ok.clear();
         HIP_TRY(c, dalloc(&b->d_setqs, (size_t)b->rs.cap * 12));
+        HIP_TRY(c, dalloc(&b->d_eigmode, (size_t)b->rs.cap));
+        HIP_TRY(c, dalloc(&b->d_eigref, (size_t)b->rs.cap * 4096));
         b->setqs_cap = (size_t)b->rs.cap * 12;
       }
-      HIP_TRY(c, hipMemcpyAsync(b->d_setqs, sm->qs + k0 * 12, nk * 12 * 8, hipMemcpyHostToDevice,
-                                c->stream));
-      HIP_TRY(c, launch_omega_eigen(b->d_setqs, (int)nk, b->rs, c->stream));
+      // per set: keep (inactive, or settings as last built), warm (the
+      // reference basis has the same pi: only kappa / omega differ), or cold
+      if (one_chunk && b->eig_ok.size() != nk) {
+        b->eig_ok.assign(nk, 0);
+        b->eig_qs.assign(nk * 12, 0.0);
+      }
+      if (one_chunk && b->eig_ref_ok.size() != nk) {
+        b->eig_ref_ok.assign(nk, 0);
+        b->eig_ref_qs.assign(nk * 12, 0.0);
+      }
+      eigmode.assign(nk, kEigCold);
+      bool any = false;
+      for (size_t k = 0; k < nk; ++k) {
+        const double* q = sm->qs + (k0 + k) * 12;
+        if (!act[k]) {
+          eigmode[k] = kEigKeep;
+          continue;
+        }
+        if (one_chunk && b->eig_ok[k] && std::memcmp(&b->eig_qs[k * 12], q, 12 * 8) == 0)
+          eigmode[k] = kEigKeep;
+        else if (one_chunk && b->eig_ref_ok[k] &&
+                 std::memcmp(&b->eig_ref_qs[k * 12 + 2], q + 2, 10 * 8) == 0)
+          eigmode[k] = kEigWarm;
+        any = any || eigmode[k] != kEigKeep;
+      }
+      if (any) {
+        if (one_chunk)  // invalid until the launch below is issued
+          for (size_t k = 0; k < nk; ++k) {
+            if (eigmode[k] != kEigKeep) b->eig_ok[k] = 0;
+            if (eigmode[k] == kEigCold) b->eig_ref_ok[k] = 0;
+          }
+        HIP_TRY(c, hipMemcpyAsync(b->d_setqs, sm->qs + k0 * 12, nk * 12 * 8,
+                                  hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(b->d_eigmode, eigmode.data(), nk, hipMemcpyHostToDevice,
+                                  c->stream));
+        HIP_TRY(c, launch_omega_eigen(b->d_setqs, b->d_eigmode, b->d_eigref, (int)nk, b->rs,
+                                      c->stream));
+        if (one_chunk)
+          for (size_t k = 0; k < nk; ++k) {
+            const double* q = sm->qs + (k0 + k) * 12;
+            if (eigmode[k] != kEigKeep) {
+              std::memcpy(&b->eig_qs[k * 12], q, 12 * 8);
+              b->eig_ok[k] = 1;
+            }
+            if (eigmode[k] == kEigCold) {
+              std::memcpy(&b->eig_ref_qs[k * 12], q, 12 * 8);
+              b->eig_ref_ok[k] = 1;
+            }
+          }
+      }
     }
     float tp = 0.f;
     rc = run_pbuild(m, b->rs, nr * nm, rho + (size_t)r0 * nm, fmt, &tp,
@@ -437,6 +498,8 @@ int adjoint_regions(pcsf_batch* b, const double* rho, const uint8_t* active, dou
   cap = std::min<int>(cap, std::max(nm, b->nreg * nm));
   if (b->rs.cap < cap || (!tangent && !b->rs.fp64) || (tangent && !b->rs.d_Dfrag) || !keep) {
     free_psets(b->rs);
+    b->eig_ok.clear();
+    b->eig_ref_ok.clear();
     HIP_TRY(c, alloc_psets(b->rs, cap, m->nbr, want_fp64, 0, want_tangent));
   }
   const int chunk_regions = b->rs.cap / nm;
@@ -1023,6 +1086,8 @@ void pcsf_batch_destroy(pcsf_batch* b) {
   dfree(b->d_gsq);
   dfree(b->d_adj_scratch);
   dfree(b->d_setqs);
+  dfree(b->d_eigmode);
+  dfree(b->d_eigref);
   free_psets(b->rs);
   for (auto& ev : b->ev)
     if (ev) (void)hipEventDestroy(ev);

@@ -180,6 +180,13 @@ struct pcsf_batch {
   size_t adj_scratch = 0;           // its size in doubles
   double* d_setqs = nullptr;        // per-set model settings of a chunk (omega test)
   size_t setqs_cap = 0;             // its size in doubles
+  uint8_t* d_eigmode = nullptr;     // per set: kEigCold / kEigWarm / kEigKeep
+  double* d_eigref = nullptr;       // per set: the reference basis of its warm starts
+  // the settings each set of `rs` was last built from and those of its
+  // reference basis (host), valid while a pass fits in one chunk (set k is
+  // pair k) and `rs` is not reallocated
+  std::vector<double> eig_qs, eig_ref_qs;
+  std::vector<uint8_t> eig_ok, eig_ref_ok;
 };
 
 // Per-region-scale launch description (one chunk of regions).
@@ -260,7 +267,12 @@ hipError_t launch_forward_fp64m(const pcsf_model* m, const pcsf_batch* b, const 
 // the omega test's rate model of every active set of a chunk from its twelve
 // settings (d_qs: nsets * 12): eigensystem and root prior into the sets'
 // own-model arrays
-hipError_t launch_omega_eigen(const double* d_qs, int nsets, const PSets& sets, hipStream_t s);
+// Per-set modes of launch_omega_eigen: build from scratch (and store the
+// eigenvectors as the set's reference basis in d_ref, 4096 doubles a set),
+// start from that basis (same pi), or keep the outputs.
+constexpr uint8_t kEigCold = 0, kEigWarm = 1, kEigKeep = 2;
+hipError_t launch_omega_eigen(const double* d_qs, const uint8_t* d_mode, double* d_ref,
+                              int nsets, const PSets& sets, hipStream_t s);
 hipError_t launch_prune_tangent_regions(const pcsf_model* m, const pcsf_batch* b,
                                         const RegionChunk& ch, hipStream_t s);
 // sums over each region of the chunk into loglik / elpr (nreg * nm)

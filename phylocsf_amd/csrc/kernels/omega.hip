@@ -15,6 +15,15 @@
 //        is below 1e-30 of the total or stops falling (it converges
 //        quadratically to a rounding floor near 1e-31: eight sweeps for the
 //        omega models), at most 64 sweeps; V = Pi^-1/2 U and V^-1 = U^T Pi^1/2.
+//        A cold build also stores U as the set's reference basis U0; a later
+//        build of the same set with the same pi (kappa or omega changed)
+//        starts from it: A' = U0^T A U0 is near-diagonal, the rotations go on
+//        from U = U0, and the stopping rule is unchanged (a few sweeps instead
+//        of eight; Golub & Van Loan on Jacobi for nearby matrices).  Starting
+//        from the fixed U0, not from the previous build, keeps the result a
+//        function of the settings and U0's (the fit's starting point), not of
+//        the path between them.  A set whose settings did not change keeps its
+//        eigensystem.
 //
 // One 256-thread workgroup per set, A and U in LDS (rows padded to 65
 // doubles).  Outputs in the sets' own-model layout: V transposed, V^-1, the
@@ -46,10 +55,13 @@ __device__ __forceinline__ double exchangeability(const double* qs, int i, int j
   return kp * op;
 }
 
+// mode per set: kEigCold builds from scratch and stores U in ref (64 x 64
+// per set), kEigWarm starts from ref, kEigKeep leaves the outputs (unchanged
+// settings, or an inactive set)
 __global__ __launch_bounds__(256) void omega_eigen_kernel(
-    const double* __restrict__ qs_all, const uint8_t* __restrict__ active, double* __restrict__ VT,
-    double* __restrict__ Vinv, double* __restrict__ lam, double* __restrict__ pi_out,
-    double* __restrict__ lnpi_out) {
+    const double* __restrict__ qs_all, const uint8_t* __restrict__ mode, double* __restrict__ ref,
+    double* __restrict__ VT, double* __restrict__ Vinv, double* __restrict__ lam,
+    double* __restrict__ pi_out, double* __restrict__ lnpi_out) {
   __shared__ double A[64 * kLd], U[64 * kLd];
   __shared__ double pi[64], sq[64], diag[64];
   __shared__ double cs[32], sn[32];
@@ -57,7 +69,8 @@ __global__ __launch_bounds__(256) void omega_eigen_kernel(
   __shared__ double red[256];
   __shared__ double total_s, scale_s;
   const int set = blockIdx.x, tid = threadIdx.x;
-  if (!active[set]) return;  // uniform over the workgroup
+  const int md = mode[set];
+  if (md == kEigKeep) return;  // uniform over the workgroup
   const double* qs = qs_all + (size_t)set * 12;
   // codon frequencies (OmegaModel.ml:24-40)
   if (tid < 64) {
@@ -114,12 +127,56 @@ __global__ __launch_bounds__(256) void omega_eigen_kernel(
     U[i * kLd + j] = 0.5 * (sq[i] * qij / sq[j] + sq[j] * qji / sq[i]);
   }
   __syncthreads();
+  double* vt = VT + (size_t)set * 4096;
+  double* vi = Vinv + (size_t)set * 4096;
+  double* u0 = ref + (size_t)set * 4096;
   for (int e = tid; e < 4096; e += 256) {
     const int i = e >> 6, j = e & 63;
     const double x = U[i * kLd + j];
     A[i * kLd + j] = x;
-    U[i * kLd + j] = (i == j) ? 1.0 : 0.0;
+    U[i * kLd + j] = (md == kEigWarm) ? u0[e] : (i == j) ? 1.0 : 0.0;
     part = fma(x, x, part);
+  }
+  if (md == kEigWarm) {
+    // A <- U0^T (A U0), each product through registers (16 entries per
+    // thread) so that no third LDS matrix is needed
+    double t[16];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int e = tid + 256 * k, i = e >> 6, j = e & 63;
+      double x = 0.0;
+      for (int l = 0; l < 64; ++l) x = fma(A[i * kLd + l], U[l * kLd + j], x);
+      t[k] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int e = tid + 256 * k;
+      A[(e >> 6) * kLd + (e & 63)] = t[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int e = tid + 256 * k, i = e >> 6, j = e & 63;
+      double x = 0.0;
+      for (int l = 0; l < 64; ++l) x = fma(U[l * kLd + i], A[l * kLd + j], x);
+      t[k] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int e = tid + 256 * k, i = e >> 6, j = e & 63;
+      A[i * kLd + j] = t[k];
+    }
+    __syncthreads();
+    if (tid < 64) {  // symmetric up to rounding: the mean of the two halves
+      for (int j = tid + 1; j < 64; ++j) {
+        const double m = 0.5 * (A[tid * kLd + j] + A[j * kLd + tid]);
+        A[tid * kLd + j] = m;
+        A[j * kLd + tid] = m;
+      }
+    }
   }
   red[tid] = part;
   __syncthreads();
@@ -200,14 +257,15 @@ __global__ __launch_bounds__(256) void omega_eigen_kernel(
       __syncthreads();
     }
   }
-  // outputs: V^T[m][i] = U[i][m] / sq[i], V^-1[m][i] = U[i][m] sq[i]
-  double* vt = VT + (size_t)set * 4096;
-  double* vi = Vinv + (size_t)set * 4096;
+  // outputs: V^T[m][i] = U[i][m] / sq[i], V^-1[m][i] = U[i][m] sq[i]; a
+  // cold build's U is the set's reference basis
+  __syncthreads();
   for (int e = tid; e < 4096; e += 256) {
     const int m = e >> 6, i = e & 63;
     const double u = U[i * kLd + m];
     vt[e] = u / sq[i];
     vi[e] = u * sq[i];
+    if (md == kEigCold) u0[e] = U[(e >> 6) * kLd + (e & 63)];
   }
   if (tid < 64) lam[(size_t)set * 64 + tid] = A[tid * kLd + tid];
   if (tid == 0) {
@@ -225,9 +283,10 @@ __global__ __launch_bounds__(256) void omega_eigen_kernel(
 
 }  // namespace
 
-hipError_t launch_omega_eigen(const double* d_qs, int nsets, const PSets& sets, hipStream_t s) {
+hipError_t launch_omega_eigen(const double* d_qs, const uint8_t* d_mode, double* d_ref,
+                              int nsets, const PSets& sets, hipStream_t s) {
   if (nsets <= 0) return hipSuccess;
-  hipLaunchKernelGGL(omega_eigen_kernel, dim3(nsets), dim3(256), 0, s, d_qs, sets.d_active,
+  hipLaunchKernelGGL(omega_eigen_kernel, dim3(nsets), dim3(256), 0, s, d_qs, d_mode, d_ref,
                      sets.d_eVT, sets.d_eVinv, sets.d_elam, sets.d_epi, sets.d_elnpi);
   return hipGetLastError();
 }
