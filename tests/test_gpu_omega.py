@@ -115,6 +115,31 @@ def test_omega_mfma_precisions(ctx, params_dir):
         assert d.max() <= 0.05
 
 
+def test_omega_warm_started_eigensystems(ctx, params_dir, monkeypatch):
+    """With every (region, hypothesis) set in one chunk the rate models'
+    eigensystems start from each set's first one (and unchanged sets keep
+    theirs); with a 1 MiB set budget every region is a chunk of its own and
+    every eigensystem is built cold.  Both follow the same maximiser, so they
+    agree as the oracle comparison does: estimates to 1e-4 relative, MAP lpr
+    to 1e-8 relative, scores to 1e-5 dB + DB 1e-8 (|lpr_H0| + |lpr_H1|)."""
+    _say("omega test, warm-started against cold eigensystems")
+    codes, off = _sim_regions("29mammals", np.array([90, 14, 150, 60, 33, 120]), 11)
+    tree = P.newick_tree(open(os.path.join(params_dir, "29mammals.nh")).read())
+    m = P.PhyloCSFModel.tree_only(ctx, tree)
+    monkeypatch.delenv("PHYLOCSF_SET_BUDGET_MB", raising=False)
+    warm = P.score_omega(P.Batch(m, codes, off), precision=P.PCSF_FP64M)
+    monkeypatch.setenv("PHYLOCSF_SET_BUDGET_MB", "1")
+    cold = P.score_omega(P.Batch(m, codes, off), precision=P.PCSF_FP64M)
+    assert (warm["status"] == 0).all() and (cold["status"] == 0).all()
+    for r in range(len(off) - 1):
+        tol = 1e-5 + P.DB * 1e-8 * (abs(cold["lpr"][r, 0]) + abs(cold["lpr"][r, 1]))
+        assert abs(warm["score"][r] - cold["score"][r]) <= tol, r
+        for h in range(2):
+            assert abs(warm["lpr"][r, h] - cold["lpr"][r, h]) <= 1e-8 * abs(cold["lpr"][r, h])
+            for key in ("rho", "kappa"):
+                assert abs(warm[key][r, h] - cold[key][r, h]) <= 1e-4 * cold[key][r, h], (r, h)
+
+
 def test_tree_only_model_refuses_ecm_entry_points(ctx, params_dir):
     tree = P.newick_tree(open(os.path.join(params_dir, "12flies.nh")).read())
     m = P.PhyloCSFModel.tree_only(ctx, tree)
